@@ -1,0 +1,117 @@
+// Device-side building blocks shared by the gfx950 kernels: element types,
+// 16-byte fragment loads, the per-dtype MFMA step and the epilogue op.
+//
+// Fragment convention (all kernels): a "chunk" is one 16-byte load per lane.
+// In a 16x16 MFMA tile, lane l works on row/col (l & 15) and lane group
+// g = l >> 4 (0..3) supplies reduction elements [g*E, g*E+E) of the chunk,
+// E = 16 / sizeof(T).  A and B use the same element order, so the chunk's
+// reduction order is a permutation shared by both operands and the sum is
+// unchanged.  One chunk covers 4*E reduction elements:
+//   f32  : 16 elements -> 4 x v_mfma_f32_16x16x4_f32
+//   f16  : 32 elements -> 1 x v_mfma_f32_16x16x32_f16
+//   bf16 : 32 elements -> 1 x v_mfma_f32_16x16x32_bf16
+//   e4m3 : 64 elements -> 2 x v_mfma_f32_16x16x32_fp8_fp8
+// The 16x16 accumulator holds, in lane l, column (l & 15) and rows
+// 4*(l >> 4) + i, i = 0..3.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cs {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+struct bf16 { uint16_t bits; };
+struct e4m3 { uint8_t bits; };
+
+// ---------------------------------------------------------------- loads
+__device__ __forceinline__ u32x4 ld16(const void* p) {
+  return *reinterpret_cast<const u32x4*>(p);
+}
+__device__ __forceinline__ u32x4 zero16() { return u32x4{0u, 0u, 0u, 0u}; }
+
+// ---------------------------------------------------------------- conversions
+__device__ __forceinline__ float to_f32(float v) { return v; }
+__device__ __forceinline__ float to_f32(_Float16 v) { return (float)v; }
+__device__ __forceinline__ float to_f32(bf16 v) {
+  return __uint_as_float(((uint32_t)v.bits) << 16);
+}
+
+template <typename OutT> __device__ __forceinline__ OutT from_f32(float v);
+template <> __device__ __forceinline__ float from_f32<float>(float v) { return v; }
+template <> __device__ __forceinline__ _Float16 from_f32<_Float16>(float v) {
+  return (_Float16)v;  // v_cvt_f16_f32: round to nearest even
+}
+template <> __device__ __forceinline__ bf16 from_f32<bf16>(float v) {
+  // IEEE-754 binary32 -> bfloat16, round to nearest even; NaN stays quiet NaN.
+  uint32_t u = __float_as_uint(v);
+  bf16 r;
+  if ((u & 0x7fffffffu) > 0x7f800000u) {
+    r.bits = (uint16_t)((u >> 16) | 0x40u);
+  } else {
+    u += 0x7fffu + ((u >> 16) & 1u);
+    r.bits = (uint16_t)(u >> 16);
+  }
+  return r;
+}
+
+// ---------------------------------------------------------------- MFMA step
+template <typename T> struct Mma;
+
+template <> struct Mma<float> {
+  static constexpr int kElems = 4;   // per lane per chunk
+  static constexpr int kChunk = 16;  // reduction elements per chunk
+  __device__ __forceinline__ static void step(f32x4& acc, u32x4 a, u32x4 b) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(a.x), __uint_as_float(b.x), acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(a.y), __uint_as_float(b.y), acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(a.z), __uint_as_float(b.z), acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(a.w), __uint_as_float(b.w), acc, 0, 0, 0);
+  }
+};
+
+template <> struct Mma<_Float16> {
+  static constexpr int kElems = 8;
+  static constexpr int kChunk = 32;
+  __device__ __forceinline__ static void step(f32x4& acc, u32x4 a, u32x4 b) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a),
+                                                 __builtin_bit_cast(f16x8, b), acc, 0, 0, 0);
+  }
+};
+
+template <> struct Mma<bf16> {
+  static constexpr int kElems = 8;
+  static constexpr int kChunk = 32;
+  __device__ __forceinline__ static void step(f32x4& acc, u32x4 a, u32x4 b) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                  __builtin_bit_cast(bf16x8, b), acc, 0, 0, 0);
+  }
+};
+
+template <> struct Mma<e4m3> {
+  static constexpr int kElems = 16;
+  static constexpr int kChunk = 64;
+  __device__ __forceinline__ static void step(f32x4& acc, u32x4 a, u32x4 b) {
+    long a0 = (long)(((uint64_t)a.y << 32) | a.x), a1 = (long)(((uint64_t)a.w << 32) | a.z);
+    long b0 = (long)(((uint64_t)b.y << 32) | b.x), b1 = (long)(((uint64_t)b.w << 32) | b.z);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a0, b0, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a1, b1, acc, 0, 0, 0);
+  }
+};
+
+// ---------------------------------------------------------------- epilogue
+// D = act(alpha * acc + beta * source); act codes follow tv::gemm::Activation.
+enum : int { kActNone = 0, kActReLU = 1, kActSigmoid = 2, kActLeakyReLU = 3 };
+
+__device__ __forceinline__ float apply_act(float v, int act, float act_alpha) {
+  switch (act) {
+    case kActReLU: return v > 0.f ? v : 0.f;
+    case kActSigmoid: return 1.f / (1.f + expf(-v));
+    case kActLeakyReLU: return v >= 0.f ? v : v * act_alpha;
+    default: return v;
+  }
+}
+
+}  // namespace cs

@@ -1,0 +1,94 @@
+// Host-visible launch interface of the gfx950 kernels (no torch types).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cs {
+
+// Element type tags used by the launchers; values are tv::DType codes.
+enum DT : int {
+  kF32 = 0, kI32 = 1, kF64 = 4, kF16 = 7, kU32 = 10, kBF16 = 12, kE4M3 = 14
+};
+
+// Row-gather implicit GEMM (sparse conv forward / dgrad, gather-A scatter-C GEMM).
+//
+// Sorted row r (0 <= r < m) of the output tile space accumulates, for every
+// kernel offset f enabled in its 16-row tile mask,
+//   acc[r][n] += sum_k A[g(r,f)][k] * B_f[n][k]
+// with g(r,f) = pairs[f*pair_ld + col(r)] (or col(r) when pairs is null),
+// col(r) = pair_col ? pair_col[r] : r, and rows with g = -1 contributing zero.
+// B_f[n][k] = b[f*b_fs + n*b_ns + k].  The result row is written to
+// out_row ? out_row[r] : r as act(alpha * scale[n] * acc + beta * src).
+struct RowGatherArgs {
+  const void* a; int64_t lda;
+  const void* b; int64_t b_fs; int64_t b_ns;
+  void* c; int64_t ldc;
+  const void* d; int64_t ldd; int32_t d_is_bias;
+  const int32_t* pairs; int64_t pair_ld;
+  const int32_t* pair_col;
+  const int32_t* out_row;
+  const uint32_t* mask;      // sorted per-row masks; null -> every offset of the filter
+  uint32_t* mask_out;        // per-16-row tile masks (forward), may be null
+  const float* scale;        // per-output-column multiplier, may be null
+  int32_t m, kred, nout, kv;
+  uint32_t mask_filter;
+  int32_t reverse_mask;      // dgrad of a submanifold layer reuses forward masks
+  int32_t skip_empty;        // dgrad: tiles whose (mask & filter) is empty are not written
+  float alpha, beta, act_alpha, act_beta;
+  int32_t act_type;
+};
+
+// Row-pair outer-product reduction (sparse conv wgrad, ShuffleAB GEMM).
+//   out[f][mm][nn] = sum_r A[ra(r)][mm] * B[rb(r,f)][nn]
+// ra(r) = a_rows ? a_rows[r] : r;  rb(r,f) = pairs ? pairs[f*pair_ld + ra(r)]
+// : (b_rows ? b_rows[r] : r); row r counts for offset f only if rb >= 0 and
+// (mask == null or bit f of mask[r / mask_width]).  Rows are split into
+// `splits` contiguous ranges; each split writes an FP32 partial that a
+// fixed-order tree reduces (deterministic, no atomics).
+struct WgradArgs {
+  const void* a; int64_t lda;
+  const void* b; int64_t ldb;
+  const int32_t* a_rows;
+  const int32_t* b_rows;
+  const int32_t* pairs; int64_t pair_ld;
+  const uint32_t* mask; int32_t mask_width; int32_t num_mask;
+  int32_t nrows, M, N, kv;
+  uint32_t mask_filter;
+  int32_t splits, rows_per_split;
+  float* partial;            // [splits][kv][M][N] FP32
+};
+
+// out[mm*o_ms + f*o_fs + nn] = act(alpha * tree_sum_s partial[s][f][mm][nn] + beta * src)
+// src = src_is_bias ? src[nn] : src[mm*s_ms + f*s_fs + nn]  (src may alias out)
+struct ReduceArgs {
+  const float* partial; int32_t splits, kv, M, N;
+  void* out; int64_t o_ms, o_fs;
+  const void* src; int64_t s_ms, s_fs; int32_t src_is_bias;
+  uint32_t mask_filter;
+  float alpha, beta, act_alpha;
+  int32_t act_type;
+};
+
+// block_n: output columns per wave, 64 or 128.
+hipError_t launch_rowgather_f32(int out_dtype, int block_n, const RowGatherArgs& args, hipStream_t s);
+hipError_t launch_rowgather_f16(int out_dtype, int block_n, const RowGatherArgs& args, hipStream_t s);
+hipError_t launch_rowgather_bf16(int out_dtype, int block_n, const RowGatherArgs& args, hipStream_t s);
+hipError_t launch_rowgather_e4m3(int out_dtype, int block_n, const RowGatherArgs& args, hipStream_t s);
+hipError_t launch_rowgather(int in_dtype, int out_dtype, int block_n, const RowGatherArgs& args, hipStream_t s);
+bool rowgather_supported(int in_dtype, int out_dtype);
+int rowgather_block_n(int nout);
+hipError_t launch_wgrad(int in_dtype, const WgradArgs& args, hipStream_t s);
+hipError_t launch_reduce(int out_dtype, const ReduceArgs& args, hipStream_t s);
+// out[f][c][r] = in[r][f][c] for a [R, F, Cc] tensor of `elem_bytes`-byte elements.
+hipError_t launch_transpose(const void* in, void* out, int64_t R, int64_t F, int64_t Cc,
+                            int64_t in_row_stride, int elem_bytes, hipStream_t s);
+// 2-D copy with strides (elements), used to make operands dense.
+hipError_t launch_zero(void* p, int64_t nbytes, hipStream_t s);
+
+constexpr int kRowsPerWave = 64;     // four 16-row mask tiles per wave
+constexpr int kWavesPerBlock = 4;
+constexpr int kMaskTile = 16;        // rows per kernel-offset mask tile
+constexpr int kWgradRows = 64;       // rows staged per wgrad step
+constexpr int kWgradTile = 64;       // wgrad output tile (M and N)
+
+}  // namespace cs

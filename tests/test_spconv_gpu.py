@@ -1,0 +1,166 @@
+"""GPU parity of ConvMainUnitTest.implicit_gemm2 (mask-sparse forward, dgrad,
+wgrad) against the FP64 oracle, called through the C ABI.
+
+Tolerances (relative Frobenius error vs the FP64 oracle evaluated on the
+*quantised* inputs, i.e. only accumulation and output rounding count):
+f32 2e-6, f16 1e-3, bf16 8e-3, e4m3 inputs with f16 output 1e-3.
+Index outputs (mask_output) are compared bit-exact.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import spconv_ref as ref
+from spconv_case import (ALL, DT, TORCH_DT, conv_params, desp_for, idx_dev, quantize,
+                         run_fwd, subm_case, to_dev)
+
+from cumm_amd import pointcloud as pc
+from cumm_amd.conv.main import ConvMainUnitTest
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"f32": 2e-6, "f16": 1e-3, "bf16": 8e-3, "e4m3": 1e-3}
+
+
+def _rel(y, y64):
+    return ref.relative_error(y, y64)[1]
+
+
+@pytest.fixture(scope="module")
+def case64():
+    return subm_case(3000, 64, 64, seed=3)
+
+
+@pytest.mark.parametrize("dt", ["f32", "f16", "bf16"])
+def test_forward_parity(case64, dt):
+    c = case64
+    print(f"[fwd] dt={dt} N={c.prob.n} C={c.C} K={c.K}", flush=True)
+    out, mo, desp = run_fwd(c, dt)
+    exp, tm = ref.conv_forward(quantize(c.X, dt), quantize(c.W, dt), c.prob.pair_fwd, c.prob.mask_sorted,
+                               c.prob.argsort, desp.tile_shape[0])
+    exp = quantize(exp, dt)  # output rounding of the reference value
+    assert np.array_equal(mo, tm), "mask_output must be bit-exact"
+    assert _rel(out, exp) <= TOL[dt]
+
+
+@pytest.mark.parametrize("C,K", [(16, 32), (32, 64), (64, 128), (128, 128), (48, 80)])
+def test_forward_channels(C, K):
+    c = subm_case(1000 + C, C, K, seed=C + K)
+    print(f"[fwd-ch] C={C} K={K}", flush=True)
+    out, mo, desp = run_fwd(c, "f16")
+    exp, tm = ref.conv_forward(quantize(c.X, "f16"), quantize(c.W, "f16"), c.prob.pair_fwd,
+                               c.prob.mask_sorted, c.prob.argsort, desp.tile_shape[0])
+    assert np.array_equal(mo, tm)
+    assert _rel(out, quantize(exp, "f16")) <= TOL["f16"]
+
+
+def test_forward_epilogue_bias_beta_act(case64):
+    c = case64
+    n, K = c.prob.n, c.K
+    rng = np.random.default_rng(7)
+    init = rng.standard_normal((n, K)).astype(np.float32)
+    bias = rng.standard_normal(K).astype(np.float32)
+    desp = desp_for(0, "f32", c.C, K)
+    for beta, use_bias, act, aa in [(1.0, False, 0, 0.0), (0.5, True, 1, 0.0), (1.0, False, 3, 0.1),
+                                    (2.0, True, 2, 0.0)]:
+        print(f"[fwd-epi] beta={beta} bias={use_bias} act={act}", flush=True)
+        out = torch.from_numpy(init.copy()).cuda()
+        bt = torch.from_numpy(bias).cuda() if use_bias else None
+        p = conv_params(0, desp, to_dev(c.X, "f32"), to_dev(c.W, "f32"), out, c.prob.pair_fwd,
+                        c.prob.mask_sorted, c.prob.argsort, alpha=0.75, beta=beta, bias=bt,
+                        act_type=act, act_alpha=aa)
+        ConvMainUnitTest.implicit_gemm2(p)
+        torch.cuda.synchronize()
+        exp, _ = ref.conv_forward(c.X, c.W, c.prob.pair_fwd, c.prob.mask_sorted, c.prob.argsort,
+                                  desp.tile_shape[0], alpha=0.75, beta=beta,
+                                  bias=bias if use_bias else None, out_init=init, act_type=act,
+                                  act_alpha=aa)
+        assert _rel(out.double().cpu().numpy(), exp) <= 5e-6
+
+
+def test_forward_split_masks_two_launches(case64):
+    """The reference's split-mask pattern: offsets 0..13 then 14..26, second
+    launch with beta=1 (cumm/conv/main_real_spconv.py:127-143, 275-294)."""
+    c = case64
+    desp = desp_for(0, "f32", c.C, c.K)
+    out = torch.zeros((c.prob.n, c.K), dtype=torch.float32, device="cuda")
+    exp = None
+    tiles = (c.prob.n + 15) // 16
+    for j, split in enumerate([0b11111111111111, 0b1111111111111 << 14]):
+        ms, order = pc.sort_masks(c.prob.mask, split)
+        mo = torch.zeros(tiles, dtype=torch.int32, device="cuda")
+        p = conv_params(0, desp, to_dev(c.X, "f32"), to_dev(c.W, "f32"), out, c.prob.pair_fwd, ms, order,
+                        mask_output=mo, beta=float(j))
+        ConvMainUnitTest.implicit_gemm2(p)
+        torch.cuda.synchronize()
+        exp, tm = ref.conv_forward(c.X, c.W, c.prob.pair_fwd, ms, order, 16, beta=float(j), out_init=exp)
+        assert np.array_equal(mo.cpu().numpy().view(np.uint32), tm)
+    full, _ = ref.conv_forward(c.X, c.W, c.prob.pair_fwd, c.prob.mask_sorted, c.prob.argsort, 16)
+    assert _rel(out.double().cpu().numpy(), exp) <= 5e-6
+    assert _rel(exp, full) <= 1e-12
+
+
+@pytest.mark.parametrize("dt", ["f32", "f16", "bf16"])
+@pytest.mark.parametrize("reverse", [True, False])
+def test_dgrad_parity(case64, dt, reverse):
+    c = case64
+    prob = c.prob
+    print(f"[dgrad] dt={dt} reverse={reverse}", flush=True)
+    desp = desp_for(1, dt, c.C, c.K)
+    if reverse:
+        mask, order = prob.mask_sorted, prob.argsort  # forward masks, reversed in-kernel
+    else:
+        mask, order = pc.sort_masks(pc.row_masks(prob.pair_bwd))
+    din = torch.zeros((prob.n, c.C), dtype=TORCH_DT[dt], device="cuda")
+    p = conv_params(1, desp, din, to_dev(c.W, dt), to_dev(c.dO, dt), prob.pair_bwd, mask, order,
+                    reverse_mask=reverse)
+    ConvMainUnitTest.implicit_gemm2(p)
+    torch.cuda.synchronize()
+    exp = ref.conv_dgrad(quantize(c.dO, dt), quantize(c.W, dt), prob.pair_bwd, mask, order, 16,
+                         reverse_mask=reverse)
+    assert _rel(din.double().cpu().numpy(), quantize(exp, dt)) <= TOL[dt]
+
+
+@pytest.mark.parametrize("dt,out_dt", [("f32", "f32"), ("f16", "f16"), ("bf16", "bf16"), ("f16", "f32")])
+@pytest.mark.parametrize("split_k", [1, 7, 32])
+def test_wgrad_parity(case64, dt, out_dt, split_k):
+    c = case64
+    prob = c.prob
+    print(f"[wgrad] dt={dt} out={out_dt} split={split_k}", flush=True)
+    _, mo, fdesp = run_fwd(c, "f32")
+    desp = desp_for(2, dt, c.C, c.K, out_dt)
+    dw = torch.zeros((c.K, prob.kv, c.C), dtype=TORCH_DT[out_dt], device="cuda")
+    p = conv_params(2, desp, to_dev(c.X, dt), dw, to_dev(c.dO, dt), prob.pair_fwd, mo, prob.argsort,
+                    mask_width=fdesp.tile_shape[0], split_k=split_k)
+    ConvMainUnitTest.implicit_gemm2(p)
+    torch.cuda.synchronize()
+    exp = ref.conv_wgrad(quantize(c.X, dt), quantize(c.dO, dt), prob.pair_fwd, mo, prob.argsort,
+                         fdesp.tile_shape[0])
+    got = dw.double().cpu().numpy()
+    assert _rel(got, quantize(exp, out_dt)) <= max(TOL[dt], TOL[out_dt]) * 2
+
+
+def test_wgrad_deterministic(case64):
+    c = case64
+    prob = c.prob
+    _, mo, fdesp = run_fwd(c, "f32")
+    desp = desp_for(2, "f16", c.C, c.K)
+    res = []
+    for _ in range(3):
+        dw = torch.zeros((c.K, prob.kv, c.C), dtype=torch.float16, device="cuda")
+        p = conv_params(2, desp, to_dev(c.X, "f16"), dw, to_dev(c.dO, "f16"), prob.pair_fwd, mo,
+                        prob.argsort, mask_width=16, split_k=16)
+        ConvMainUnitTest.implicit_gemm2(p)
+        torch.cuda.synchronize()
+        res.append(dw.cpu().view(torch.int16).numpy())
+    assert np.array_equal(res[0], res[1]) and np.array_equal(res[0], res[2])
+
+
+def test_ragged_and_tiny():
+    for n in (1, 15, 17, 63, 65, 257):
+        c = subm_case(n, 32, 32, seed=n, grid=(4, 16, 16))
+        print(f"[ragged] N={n}", flush=True)
+        out, mo, desp = run_fwd(c, "f32")
+        exp, tm = ref.conv_forward(c.X, c.W, c.prob.pair_fwd, c.prob.mask_sorted, c.prob.argsort, 16)
+        assert np.array_equal(mo, tm)
+        assert _rel(out, exp) <= 5e-6
