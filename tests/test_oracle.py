@@ -1,0 +1,277 @@
+"""Pins the CPU oracle (oracle/) and the host pair builder before anything is
+compared against them.  Each check is chosen so a plausible mistake (dropped
+term, wrong sign or index, transposed operand, wrong mask granularity) fails."""
+import ctypes
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import bruteforce, gemm_ref, spconv_ref as ref
+from cumm_amd import pointcloud as pc
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def _tiny(n=220, grid=(6, 20, 20), seed=0, C=8, K=12):
+    coords = pc.plane_scene(n, grid=grid, seed=seed)
+    prob = pc.build_subm(coords, grid)
+    rng = np.random.default_rng(seed + 7)
+    X = rng.standard_normal((n, C))
+    W = rng.standard_normal((K, 27, C))
+    dO = rng.standard_normal((n, K))
+    return prob, X, W, dO
+
+
+# --------------------------------------------------------------- pair builder
+def test_subm_pair_invariants():
+    prob, *_ = _tiny()
+    pf, n = prob.pair_fwd, prob.n
+    assert np.array_equal(pf[13], np.arange(n)), "centre offset is the identity"
+    for f in range(27):
+        i = np.nonzero(pf[f] >= 0)[0]
+        j = pf[f][i]
+        # subm symmetry: pair[f][i] = j  <=>  pair[26 - f][j] = i
+        assert np.array_equal(pf[26 - f][j], i)
+    m = pc.row_masks(pf)
+    for f in range(27):
+        assert np.array_equal(((m >> f) & 1).astype(bool), pf[f] >= 0)
+    assert np.array_equal(np.sort(prob.argsort), np.arange(n)), "argsort is a permutation"
+    assert np.all(np.diff(prob.mask_sorted.astype(np.int64)) >= 0), "masks grouped by key"
+    assert np.array_equal(prob.mask_sorted, prob.mask[prob.argsort])
+
+
+def test_subm_pairs_match_bruteforce_neighbours():
+    prob, *_ = _tiny(n=150, seed=3)
+    lut = {tuple(c): i for i, c in enumerate(prob.coords.tolist())}
+    offs = pc.kernel_offsets()
+    for i, c in enumerate(prob.coords.tolist()):
+        for f, o in enumerate(offs):
+            nb = (c[0], c[1] + o[0] - 1, c[2] + o[1] - 1, c[3] + o[2] - 1)
+            assert prob.pair_fwd[f][i] == lut.get(nb, -1)
+
+
+def test_strided_pairs_inverse_and_bruteforce():
+    coords = pc.plane_scene(200, grid=(6, 20, 20), seed=5)
+    sp = pc.build_strided(coords, (6, 20, 20))
+    for f in range(27):
+        o = np.nonzero(sp.pair_fwd[f] >= 0)[0]
+        assert np.array_equal(sp.pair_bwd[f][sp.pair_fwd[f][o]], o)
+        assert np.count_nonzero(sp.pair_bwd[f] >= 0) == o.size
+    # every output has at least one input; every input reaches an output
+    assert np.all(pc.row_masks(sp.pair_fwd) != 0)
+    rng = np.random.default_rng(1)
+    X = rng.standard_normal((200, 4))
+    W = rng.standard_normal((5, 27, 4))
+    out, _ = ref.conv_forward(X, W, sp.pair_fwd, sp.mask_fwd_sorted, sp.argsort_fwd, 16)
+    bf = bruteforce.conv_forward_coords(coords, X, W, sp.out_coords, stride=2, padding=1, subm=False)
+    np.testing.assert_allclose(out, bf, rtol=1e-12, atol=1e-12)
+
+
+# --------------------------------------------------------------- forward
+def test_forward_matches_bruteforce():
+    prob, X, W, _ = _tiny()
+    out, tm = ref.conv_forward(X, W, prob.pair_fwd, prob.mask_sorted, prob.argsort, 16)
+    bf = bruteforce.conv_forward_coords(prob.coords, X, W)
+    np.testing.assert_allclose(out, bf, rtol=1e-12, atol=1e-12)
+    assert np.array_equal(tm, pc.tile_masks(prob.mask_sorted, 16))
+
+
+def test_forward_matches_reference_cpu_path():
+    prob, X, W, _ = _tiny(seed=4)
+    pairs, nums = ref.compact_pairs(prob.pair_fwd)
+    out, _ = ref.conv_forward(X, W, prob.pair_fwd, prob.mask_sorted, prob.argsort, 32)
+    cpu = ref.cpu_fwd(X.astype(np.float32), W.astype(np.float32), pairs, nums, prob.n)
+    np.testing.assert_allclose(cpu, out, rtol=1e-4, atol=1e-4)
+
+
+def test_forward_centre_only_is_dense_gemm():
+    n, C, K = 100, 6, 5
+    rng = np.random.default_rng(2)
+    X, W = rng.standard_normal((n, C)), rng.standard_normal((K, 27, C))
+    pair = np.full((27, n), -1, np.int32)
+    pair[13] = np.arange(n)
+    ms, order = pc.sort_masks(pc.row_masks(pair))
+    out, _ = ref.conv_forward(X, W, pair, ms, order, 16)
+    np.testing.assert_allclose(out, X @ W[:, 13, :].T, rtol=1e-12)
+    # a row permutation of the pair column changes nothing
+    perm = np.random.default_rng(3).permutation(n).astype(np.int32)
+    out2, _ = ref.conv_forward(X, W, pair, ms, perm, 16)
+    np.testing.assert_allclose(out2, out, rtol=1e-12)
+
+
+def test_forward_all_missing_is_zero_and_written():
+    n, C, K = 40, 4, 3
+    pair = np.full((27, n), -1, np.int32)
+    init = np.full((n, K), 7.0)
+    out, tm = ref.conv_forward(np.ones((n, C)), np.ones((K, 27, C)), pair, np.zeros(n, np.uint32),
+                               np.arange(n, dtype=np.int32), 16, out_init=init)
+    assert np.all(out == 0.0) and np.all(tm == 0)
+    out, _ = ref.conv_forward(np.ones((n, C)), np.ones((K, 27, C)), pair, np.zeros(n, np.uint32),
+                              np.arange(n, dtype=np.int32), 16, out_init=init, beta=1.0)
+    assert np.all(out == 7.0)
+
+
+def test_forward_tile_mask_granularity():
+    """A row whose own mask lacks f still gets f when a tile-mate has it
+    (kmask is the OR over the tile, cumm/conv/kernel.py:843-866)."""
+    n, C, K = 2, 3, 2
+    pair = np.stack([np.arange(n, dtype=np.int32)] * 27)
+    mask = np.array([1 << 0, 1 << 1], np.uint32)  # inconsistent on purpose
+    order = np.arange(n, dtype=np.int32)
+    rng = np.random.default_rng(0)
+    X, W = rng.standard_normal((n, C)), rng.standard_normal((K, 27, C))
+    out, tm = ref.conv_forward(X, W, pair, mask, order, 16)
+    exp = X @ (W[:, 0, :] + W[:, 1, :]).T
+    np.testing.assert_allclose(out, exp, rtol=1e-12)
+    assert tm.tolist() == [3]
+    out1, _ = ref.conv_forward(X, W, pair, mask, order, 1)  # tile of one row: own mask only
+    np.testing.assert_allclose(out1, np.stack([X[0] @ W[:, 0, :].T, X[1] @ W[:, 1, :].T]), rtol=1e-12)
+    outf, tmf = ref.conv_forward(X, W, pair, mask, order, 16, mask_filter=1)
+    np.testing.assert_allclose(outf, X @ W[:, 0, :].T, rtol=1e-12)
+    assert tmf.tolist() == [1]
+
+
+def test_forward_epilogue():
+    prob, X, W, _ = _tiny(seed=6)
+    rng = np.random.default_rng(9)
+    init = rng.standard_normal((prob.n, W.shape[0]))
+    bias = rng.standard_normal(W.shape[0])
+    plain, _ = ref.conv_forward(X, W, prob.pair_fwd, prob.mask_sorted, prob.argsort, 16)
+    got, _ = ref.conv_forward(X, W, prob.pair_fwd, prob.mask_sorted, prob.argsort, 16, alpha=0.5, beta=2.0,
+                              out_init=init)
+    np.testing.assert_allclose(got, 0.5 * plain + 2.0 * init, rtol=1e-12)
+    got, _ = ref.conv_forward(X, W, prob.pair_fwd, prob.mask_sorted, prob.argsort, 16, beta=1.0, bias=bias,
+                              act_type=1)
+    np.testing.assert_allclose(got, np.maximum(plain + bias, 0), rtol=1e-12)
+    got, _ = ref.conv_forward(X, W, prob.pair_fwd, prob.mask_sorted, prob.argsort, 16, act_type=3,
+                              act_alpha=0.1)
+    np.testing.assert_allclose(got, np.where(plain >= 0, plain, 0.1 * plain), rtol=1e-12)
+    got, _ = ref.conv_forward(X, W, prob.pair_fwd, prob.mask_sorted, prob.argsort, 16, act_type=2)
+    np.testing.assert_allclose(got, 1 / (1 + np.exp(-plain)), rtol=1e-12)
+
+
+# --------------------------------------------------------------- dgrad / wgrad
+def test_dgrad_adjoint_and_cpu_path():
+    prob, X, W, dO = _tiny(seed=8)
+    y, _ = ref.conv_forward(X, W, prob.pair_fwd, prob.mask_sorted, prob.argsort, 16)
+    din = ref.conv_dgrad(dO, W, prob.pair_bwd, prob.mask_sorted, prob.argsort, 16, reverse_mask=True)
+    # <conv(X), dO> == <X, conv^T(dO)>
+    assert np.isclose(np.sum(y * dO), np.sum(X * din), rtol=1e-11)
+    mb, ab = pc.sort_masks(pc.row_masks(prob.pair_bwd))
+    din2 = ref.conv_dgrad(dO, W, prob.pair_bwd, mb, ab, 16)
+    np.testing.assert_allclose(din2, din, rtol=1e-12, atol=1e-12)
+    pairs, nums = ref.compact_pairs(prob.pair_fwd)
+    cpu = ref.cpu_dgrad(dO.astype(np.float32), W.astype(np.float32), pairs, nums, prob.n)
+    np.testing.assert_allclose(cpu, din, rtol=1e-4, atol=1e-4)
+
+
+def test_dgrad_skips_empty_tiles():
+    n, C, K = 40, 3, 2
+    pair = np.full((27, n), -1, np.int32)
+    pair[13, :16] = np.arange(16)
+    mask = pc.row_masks(pair)
+    ms, order = pc.sort_masks(mask)  # the 24 empty rows sort first
+    init = np.full((n, C), 5.0)
+    din = ref.conv_dgrad(np.ones((n, K)), np.ones((K, 27, C)), pair, ms, order, 16, din_init=init)
+    written = order[np.repeat(ref.tile_masks(ms, n, 16) != 0, 16)[:n]]
+    untouched = np.setdiff1d(np.arange(n), written)
+    assert untouched.size > 0 and np.all(din[untouched] == 5.0)
+
+
+def test_wgrad_adjoint_and_cpu_path():
+    prob, X, W, dO = _tiny(seed=10)
+    tm = ref.tile_masks(prob.mask_sorted, prob.n, 16).astype(np.uint32)
+    dW = ref.conv_wgrad(X, dO, prob.pair_fwd, tm, prob.argsort, 16)
+    y, _ = ref.conv_forward(X, W, prob.pair_fwd, prob.mask_sorted, prob.argsort, 16)
+    # d<conv(X; W), dO>/dW == wgrad, and the form is linear in W
+    assert np.isclose(np.sum(y * dO), np.sum(W * dW), rtol=1e-11)
+    pairs, nums = ref.compact_pairs(prob.pair_fwd)
+    cpu = ref.cpu_wgrad(X.astype(np.float32), dO.astype(np.float32), pairs, nums, W.shape[0], X.shape[1])
+    np.testing.assert_allclose(cpu, dW, rtol=1e-4, atol=1e-3)
+
+
+def test_wgrad_reduced_mask_skip_and_filter():
+    prob, X, W, dO = _tiny(seed=12)
+    tm = ref.tile_masks(prob.mask_sorted, prob.n, 16).astype(np.uint32)
+    full = ref.conv_wgrad(X, dO, prob.pair_fwd, tm, prob.argsort, 16)
+    cut = tm.copy()
+    cut[0] &= ~np.uint32(1 << 13)  # tile 0 no longer contributes to the centre offset
+    part = ref.conv_wgrad(X, dO, prob.pair_fwd, cut, prob.argsort, 16)
+    rows = prob.argsort[:16]
+    delta = dO[rows].T @ X[prob.pair_fwd[13][rows]]
+    np.testing.assert_allclose(full[:, 13, :] - part[:, 13, :], delta, rtol=1e-10, atol=1e-10)
+    init = np.full_like(W, 3.0)
+    filt = ref.conv_wgrad(X, dO, prob.pair_fwd, tm, prob.argsort, 16, mask_filter=0b111, dW_init=init)
+    assert np.all(filt[:, 3:, :] == 3.0)
+    np.testing.assert_allclose(filt[:, :3, :], full[:, :3, :])
+    # coarser mask width: one mask word covers two 16-row tiles
+    tm32 = ref.tile_masks(prob.mask_sorted, prob.n, 32).astype(np.uint32)
+    np.testing.assert_allclose(ref.conv_wgrad(X, dO, prob.pair_fwd, tm32, prob.argsort, 32), full,
+                               rtol=1e-12, atol=1e-12)
+
+
+# --------------------------------------------------------------- gemm
+def test_gemm_closed_forms():
+    rng = np.random.default_rng(12315)
+    m, n, k = 70, 24, 36
+    a = rng.uniform(-1, 1, (m, k))
+    b = rng.uniform(-1, 1, (k, n))
+    np.testing.assert_allclose(gemm_ref.matmul2(a, b, np.zeros((m, n))), a @ b)
+    np.testing.assert_allclose(gemm_ref.matmul2(a.T.copy(), b.T.copy(), np.zeros((m, n)), trans_a=True,
+                                                trans_b=True), a @ b)
+    a_inds = rng.permutation(m).astype(np.int32)
+    c_inds = rng.permutation(m).astype(np.int32)
+    c = gemm_ref.matmul2(a, b, np.zeros((m, n)), shuffle=1, a_inds=a_inds, c_inds=c_inds)
+    exp = np.zeros((m, n))
+    exp[c_inds] = a[a_inds] @ b
+    np.testing.assert_allclose(c, exp)
+    b2 = rng.uniform(-1, 1, (m, n))
+    c = gemm_ref.matmul2(a, b2, np.zeros((k, n)), trans_a=True, shuffle=2, a_inds=a_inds, b_inds=c_inds)
+    np.testing.assert_allclose(c, a[a_inds].T @ b2[c_inds])
+    # self-cancel known answer: A.B + 1 * (-A.B) == 0
+    c = gemm_ref.matmul2(a, b, -(a @ b), beta=1.0)
+    assert np.max(np.abs(c)) < 1e-12
+    # bias broadcast
+    bias = rng.uniform(-1, 1, n)
+    np.testing.assert_allclose(gemm_ref.matmul2(a, b, np.zeros((m, n)), d=bias, beta=1.0), a @ b + bias)
+    h = gemm_ref.hilbert(5, 5)
+    assert np.isclose(h[2, 3], 1 / 6)
+
+
+def test_py_matmul_blocked_fp32():
+    rng = np.random.default_rng(19260817)
+    a = rng.uniform(-1, 1, (33, 70)).astype(np.float32)
+    b = rng.uniform(-1, 1, (70, 17)).astype(np.float32)
+    np.testing.assert_allclose(gemm_ref.py_matmul(a, b), a.astype(np.float64) @ b, rtol=1e-5, atol=1e-5)
+
+
+def _emu():
+    lib_path = ROOT / "oracle" / "_build" / "libgemm_emu.so"
+    if not lib_path.exists():
+        pytest.skip("oracle/_build/libgemm_emu.so not built (make -C oracle)")
+    L = ctypes.CDLL(str(lib_path))
+    L.gemm_emu_f32.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int64] * 3
+    return L
+
+
+def test_gemm_emulator_config1():
+    """Plain FP32 GEMM 128x128x64, U(-1,1): the CPU emulation of the FP32 kernel's
+    accumulation order against numpy FP64 (the cudasim stand-in)."""
+    L = _emu()
+    rng = np.random.default_rng(12315)
+    a = rng.uniform(-1, 1, (128, 64)).astype(np.float32)
+    b = rng.uniform(-1, 1, (64, 128)).astype(np.float32)
+    bt = np.ascontiguousarray(b.T)
+    c = np.zeros((128, 128), np.float32)
+    L.gemm_emu_f32(a.ctypes.data, bt.ctypes.data, c.ctypes.data, 128, 128, 64)
+    ref64 = a.astype(np.float64) @ b.astype(np.float64)
+    assert np.max(np.abs(c - ref64)) < 1e-4
+    # a transposed operand or a dropped chunk would fail this:
+    assert np.max(np.abs(c - (a.astype(np.float64) @ b.astype(np.float64)).T)) > 1e-2
+    # ragged k (not a multiple of 16)
+    k = 37
+    a2, b2 = a[:, :k].copy(), np.ascontiguousarray(b[:k].T)
+    c2 = np.zeros((128, 128), np.float32)
+    L.gemm_emu_f32(a2.ctypes.data, b2.ctypes.data, c2.ctypes.data, 128, 128, k)
+    assert np.max(np.abs(c2 - a2.astype(np.float64) @ b2.T.astype(np.float64))) < 1e-4
