@@ -159,6 +159,22 @@ int64_t row_stride(const cs_tensor& t) { return t.ndim >= 2 ? t.stride[0] : t.sh
 
 bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
+// Bytes from the first to one past the last element of a strided tensor.
+int64_t span_bytes(const cs_tensor& t) {
+  if (empty(t) || numel(t) == 0) return 0;
+  int64_t last = 0;
+  for (int i = 0; i < t.ndim; ++i) last += (t.shape[i] - 1) * t.stride[i];
+  return (last + 1) * dtype_bytes(t.dtype);
+}
+
+// The kernels address gathered operands with 32-bit byte offsets, the same
+// int32 limit the reference asserts (cumm/conv/nvrtc_code.py:131-143,
+// cumm/gemm/nvrtc_code.py:87-88, 104-107).
+constexpr int64_t kInt32Max = 2147483647;
+void check_int32_span(const cs_tensor& t, const char* name) {
+  CS_RT(span_bytes(t) < kInt32Max, "your data exceed int32 range (", name, ").");
+}
+
 // ------------------------------------------------------------------ desp helpers
 std::string gemm_repr(const cs_gemm_algo_desp& d) {
   // GemmAlgoDesp::__repr__  (include/tensorview/gemm/core/params.h:56-78)
@@ -334,7 +350,7 @@ const std::vector<cs_conv_algo_desp>& conv_catalogue() {
   static const std::vector<cs_conv_algo_desp> cat = [] {
     std::vector<cs_conv_algo_desp> v;
     const ConvCombo fwd_combos[] = {{0, 0, 0}, {7, 7, 7}, {12, 12, 12}, {14, 14, 7}, {14, 14, 12}};
-    for (int bn : {64, 128}) {
+    for (int bn : {64}) {
       for (auto c : fwd_combos) {
         // forward: a = input, b = weight, c = output; trans (F, T, F)
         v.push_back(conv_desp(0, rowgather_desp(c.in, c.w, c.out, cs::kMaskTile, bn, 0, 1, 0)));
@@ -358,8 +374,8 @@ const std::vector<cs_gemm_algo_desp>& gemm_catalogue() {
   static const std::vector<cs_gemm_algo_desp> cat = [] {
     std::vector<cs_gemm_algo_desp> v;
     const int dts[] = {0, 7, 12};
-    const int rows = cs::kRowsPerWave * cs::kWavesPerBlock;
-    for (int bn : {64, 128}) {
+    const int rows = cs::kRowsPerWave;
+    for (int bn : {64}) {
       for (int dt : dts) {
         for (int ta = 0; ta < 2; ++ta)
           for (int tb = 0; tb < 2; ++tb)
@@ -409,13 +425,27 @@ void acquire_workspace(Workspace& ws, const cs_tensor& given, int64_t need, hipS
 
 int64_t align256(int64_t v) { return (v + 255) / 256 * 256; }
 
-int resolve_splits(int requested, int64_t rows, int64_t blocks_per_split) {
-  if (requested > 1) return std::min(requested, 64);
-  int64_t target = 2048;
-  int64_t s = (target + std::max<int64_t>(blocks_per_split, 1) - 1) / std::max<int64_t>(blocks_per_split, 1);
-  s = std::min<int64_t>(s, std::max<int64_t>(1, (rows + 511) / 512));
-  s = std::max<int64_t>(1, std::min<int64_t>(s, 64));
-  return (int)s;
+// Pieces per offset of the planned wgrad partition: the caller's split_k_slices
+// when > 1, else a fixed default; depends only on the call's parameters.
+int resolve_splits(int requested) {
+  if (requested > 1) return std::min(requested, cs::kWgradMaxBlocks);
+  return 32;
+}
+
+constexpr int kWgradMinGroups = 8;
+
+struct WgradLayout {
+  int64_t partial = 0, glist = 0, gcount = 0, total = 0;
+};
+
+WgradLayout wgrad_layout(int64_t kv, int64_t M, int64_t N, int64_t nrows, int splits) {
+  WgradLayout w;
+  const int64_t ngroups = (nrows + 15) / 16;
+  w.partial = 0;
+  w.glist = align256(kv * splits * M * N * 4);
+  w.gcount = w.glist + align256(kv * std::max<int64_t>(ngroups, 1) * 4);
+  w.total = w.gcount + align256(kv * 4);
+  return w;
 }
 
 int popcount32(uint32_t v) { return __builtin_popcount(v); }
@@ -499,6 +529,7 @@ ConvPlan plan_conv(const cs_conv_params& p) {
          dtype_short_str(dt_out), ") do not match the algo ", conv_repr(desp));
   for (const cs_tensor* t : {&p.input, &p.weight, &p.output, &p.indices, &p.mask, &p.mask_argsort})
     check_device(*t, "tensor");
+  for (const cs_tensor* t : {&p.input, &p.weight, &p.output}) check_int32_span(*t, "conv operand");
   check_rows(p.input, "input");
   check_rows(p.output, "output");
   check_dense(p.weight, "weight");
@@ -541,10 +572,8 @@ ConvPlan plan_conv(const cs_conv_params& p) {
   if (desp.op_type == 1) {
     plan.workspace = align256(kv * C * K * dtype_bytes(dt_w));
   } else if (desp.op_type == 2) {
-    const int64_t tiles = ((K + cs::kWgradTile - 1) / cs::kWgradTile) * ((C + cs::kWgradTile - 1) / cs::kWgradTile);
-    const int nf = std::max(1, popcount32(p.mask_filter & (kv >= 32 ? 0xffffffffu : ((1u << kv) - 1u))));
-    plan.splits = resolve_splits(p.split_k_slices, N, tiles * nf);
-    plan.workspace = (int64_t)plan.splits * plan.m * plan.n * 4;
+    plan.splits = resolve_splits(p.split_k_slices);
+    plan.workspace = wgrad_layout(kv, K, C, N, plan.splits).total;
   }
   return plan;
 }
@@ -564,10 +593,12 @@ void run_conv(const cs_conv_params& p) {
     const cs_tensor& dst = plan.op == 0 ? p.output : p.input;   // scattered result
     a.a = src.data;
     a.lda = row_stride(src);
+    a.a_bytes = (uint32_t)span_bytes(src);
     if (plan.op == 0) {
       a.b = p.weight.data;        // W[K][kv][C]: B_f[n=k][c]
       a.b_fs = plan.C;
       a.b_ns = plan.kv * plan.C;
+      a.b_bytes = (uint32_t)span_bytes(p.weight);
     } else {
       // B_f[n=c][k] needs W transposed to [kv][C][K]
       CS_HIP(cs::launch_transpose(p.weight.data, ws.ptr, plan.K, plan.kv, plan.C, plan.kv * plan.C,
@@ -575,6 +606,7 @@ void run_conv(const cs_conv_params& p) {
       a.b = ws.ptr;
       a.b_fs = plan.C * plan.K;
       a.b_ns = plan.K;
+      a.b_bytes = (uint32_t)(plan.kv * plan.C * plan.K * dtype_bytes(p.weight.dtype));
     }
     a.c = dst.data;
     a.ldc = row_stride(dst);
@@ -619,15 +651,21 @@ void run_conv(const cs_conv_params& p) {
     w.N = (int32_t)plan.C;
     w.kv = (int32_t)plan.kv;
     w.mask_filter = p.mask_filter & (plan.kv >= 32 ? 0xffffffffu : ((1u << plan.kv) - 1u));
-    w.splits = plan.splits;
-    const int64_t per = (plan.N + plan.splits - 1) / plan.splits;
-    w.rows_per_split = (int32_t)((per + cs::kWgradRows - 1) / cs::kWgradRows * cs::kWgradRows);
-    w.partial = reinterpret_cast<float*>(ws.ptr);
+    w.ngroups = (int32_t)((plan.N + 15) / 16);
+    w.max_blocks = plan.splits;
+    w.min_groups = kWgradMinGroups;
+    const WgradLayout lay = wgrad_layout(plan.kv, plan.K, plan.C, plan.N, plan.splits);
+    char* base = static_cast<char*>(ws.ptr);
+    w.partial = reinterpret_cast<float*>(base + lay.partial);
+    w.glist = reinterpret_cast<int32_t*>(base + lay.glist);
+    w.gcount = reinterpret_cast<int32_t*>(base + lay.gcount);
     CS_HIP(cs::launch_wgrad(p.output.dtype, w, s));
     cs::ReduceArgs r;
     std::memset(&r, 0, sizeof(r));
     r.partial = w.partial;
-    r.splits = plan.splits;
+    r.gcount = w.gcount;
+    r.max_blocks = w.max_blocks;
+    r.min_groups = w.min_groups;
     r.kv = (int32_t)plan.kv;
     r.M = (int32_t)plan.K;
     r.N = (int32_t)plan.C;
@@ -723,6 +761,7 @@ GemmPlan plan_gemm(const cs_gemm_params& p) {
   for (const cs_tensor* t : {&p.a, &p.b, &p.c}) {
     check_device(*t, "tensor");
     check_rows(*t, "gemm operand");
+    check_int32_span(*t, "gemm operand");
   }
   if (g.d.ndim == 2) check_rows(g.d, "d");
   if (!empty(p.a_inds)) check_index_dtype(p.a_inds, "a_inds");
@@ -738,8 +777,9 @@ GemmPlan plan_gemm(const cs_gemm_params& p) {
     CS_ARG(g.m % vec == 0 && g.n % vec == 0 && row_stride(p.a) % vec == 0 && row_stride(p.b) % vec == 0,
            "m, n and row strides must be multiples of ", vec);
     const int64_t tiles = ((g.m + cs::kWgradTile - 1) / cs::kWgradTile) * ((g.n + cs::kWgradTile - 1) / cs::kWgradTile);
-    g.splits = resolve_splits(p.split_k_slices, g.k, tiles);
-    g.workspace = (int64_t)g.splits * g.m * g.n * 4;
+    (void)tiles;
+    g.splits = resolve_splits(p.split_k_slices);
+    g.workspace = wgrad_layout(1, g.m, g.n, g.k, g.splits).total;
   } else {
     CS_ARG(g.k % vec == 0, "k must be a multiple of ", vec);
     if (!trans_a) CS_ARG(row_stride(p.a) % vec == 0, "lda must be a multiple of ", vec);
@@ -775,15 +815,21 @@ void run_gemm(const cs_gemm_params& p) {
     w.N = (int32_t)g.n;
     w.kv = 1;
     w.mask_filter = 1u;
-    w.splits = g.splits;
-    const int64_t per = (g.k + g.splits - 1) / g.splits;
-    w.rows_per_split = (int32_t)((per + cs::kWgradRows - 1) / cs::kWgradRows * cs::kWgradRows);
-    w.partial = reinterpret_cast<float*>(ws.ptr);
+    w.ngroups = (int32_t)((g.k + 15) / 16);
+    w.max_blocks = g.splits;
+    w.min_groups = kWgradMinGroups;
+    const WgradLayout lay = wgrad_layout(1, g.m, g.n, g.k, g.splits);
+    char* base = static_cast<char*>(ws.ptr);
+    w.partial = reinterpret_cast<float*>(base + lay.partial);
+    w.glist = reinterpret_cast<int32_t*>(base + lay.glist);
+    w.gcount = reinterpret_cast<int32_t*>(base + lay.gcount);
     CS_HIP(cs::launch_wgrad(p.a.dtype, w, s));
     cs::ReduceArgs r;
     std::memset(&r, 0, sizeof(r));
     r.partial = w.partial;
-    r.splits = g.splits;
+    r.gcount = w.gcount;
+    r.max_blocks = w.max_blocks;
+    r.min_groups = w.min_groups;
     r.kv = 1;
     r.M = (int32_t)g.m;
     r.N = (int32_t)g.n;
@@ -822,9 +868,11 @@ void run_gemm(const cs_gemm_params& p) {
   std::memset(&a, 0, sizeof(a));
   a.a = aptr;
   a.lda = lda;
+  a.a_bytes = (uint32_t)(g.ta ? g.m * g.k * es : span_bytes(p.a));
   a.b = bptr;
   a.b_fs = 0;
   a.b_ns = ldb;
+  a.b_bytes = (uint32_t)(g.tb ? span_bytes(p.b) : g.k * g.n * es);
   a.c = p.c.data;
   a.ldc = row_stride(p.c);
   a.d = g.d.data;

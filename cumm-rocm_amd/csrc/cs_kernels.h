@@ -22,6 +22,7 @@ enum DT : int {
 struct RowGatherArgs {
   const void* a; int64_t lda;
   const void* b; int64_t b_fs; int64_t b_ns;
+  uint32_t a_bytes, b_bytes;  // extents of a and b (< 2^31): buffer-load range
   void* c; int64_t ldc;
   const void* d; int64_t ldd; int32_t d_is_bias;
   const int32_t* pairs; int64_t pair_ld;
@@ -42,9 +43,12 @@ struct RowGatherArgs {
 //   out[f][mm][nn] = sum_r A[ra(r)][mm] * B[rb(r,f)][nn]
 // ra(r) = a_rows ? a_rows[r] : r;  rb(r,f) = pairs ? pairs[f*pair_ld + ra(r)]
 // : (b_rows ? b_rows[r] : r); row r counts for offset f only if rb >= 0 and
-// (mask == null or bit f of mask[r / mask_width]).  Rows are split into
-// `splits` contiguous ranges; each split writes an FP32 partial that a
-// fixed-order tree reduces (deterministic, no atomics).
+// (mask == null or bit f of mask[r / mask_width]).
+// A plan kernel lists, per offset, the 16-row groups whose mask word has the
+// offset (glist / gcount).  Offset f's list is cut into nb_f <= max_blocks
+// contiguous pieces of at least min_groups groups; each block writes an FP32
+// partial for its piece and a fixed-order tree reduces the pieces
+// (deterministic: the partition depends only on the masks; no atomics).
 struct WgradArgs {
   const void* a; int64_t lda;
   const void* b; int64_t ldb;
@@ -54,14 +58,19 @@ struct WgradArgs {
   const uint32_t* mask; int32_t mask_width; int32_t num_mask;
   int32_t nrows, M, N, kv;
   uint32_t mask_filter;
-  int32_t splits, rows_per_split;
-  float* partial;            // [splits][kv][M][N] FP32
+  int32_t ngroups;           // div_up(nrows, 16)
+  int32_t max_blocks;        // pieces per offset (grid.z), <= 64
+  int32_t min_groups;        // groups per piece, lower bound
+  int32_t* glist;            // [kv][ngroups]
+  int32_t* gcount;           // [kv]
+  float* partial;            // [kv][max_blocks][M][N] FP32
 };
 
-// out[mm*o_ms + f*o_fs + nn] = act(alpha * tree_sum_s partial[s][f][mm][nn] + beta * src)
+// out[mm*o_ms + f*o_fs + nn] = act(alpha * tree_sum_b partial[f][b][mm][nn] + beta * src)
+// over the nb_f pieces of offset f (from gcount, as in the wgrad kernel);
 // src = src_is_bias ? src[nn] : src[mm*s_ms + f*s_fs + nn]  (src may alias out)
 struct ReduceArgs {
-  const float* partial; int32_t splits, kv, M, N;
+  const float* partial; const int32_t* gcount; int32_t max_blocks, min_groups, kv, M, N;
   void* out; int64_t o_ms, o_fs;
   const void* src; int64_t s_ms, s_fs; int32_t src_is_bias;
   uint32_t mask_filter;
@@ -77,6 +86,7 @@ hipError_t launch_rowgather_e4m3(int out_dtype, int block_n, const RowGatherArgs
 hipError_t launch_rowgather(int in_dtype, int out_dtype, int block_n, const RowGatherArgs& args, hipStream_t s);
 bool rowgather_supported(int in_dtype, int out_dtype);
 int rowgather_block_n(int nout);
+// plan + main wgrad kernels
 hipError_t launch_wgrad(int in_dtype, const WgradArgs& args, hipStream_t s);
 hipError_t launch_reduce(int out_dtype, const ReduceArgs& args, hipStream_t s);
 // out[f][c][r] = in[r][f][c] for a [R, F, Cc] tensor of `elem_bytes`-byte elements.
@@ -89,6 +99,7 @@ constexpr int kRowsPerWave = 64;     // four 16-row mask tiles per wave
 constexpr int kWavesPerBlock = 4;
 constexpr int kMaskTile = 16;        // rows per kernel-offset mask tile
 constexpr int kWgradRows = 64;       // rows staged per wgrad step
+constexpr int kWgradMaxBlocks = 64;  // pieces per offset, upper bound
 constexpr int kWgradTile = 64;       // wgrad output tile (M and N)
 
 }  // namespace cs

@@ -1,25 +1,58 @@
 // Row-gather implicit GEMM kernel (sparse conv forward / dgrad, ShuffleAC and
-// dense GEMM).  One wave owns 64 sorted rows (four 16-row mask tiles) and
-// BN = 16*NB output columns.  Gathered rows and weight rows are loaded straight
-// into MFMA operand registers (16 B per lane); no LDS and no block barrier.
-// The wave ORs its four tile masks into a scalar union and walks it with
-// s_ff1; per offset, tiles whose own mask lacks the bit are skipped with a
-// scalar branch.  Instantiated per input dtype in rg_<dtype>.hip.
+// dense GEMM).
+//
+// A block owns 64 sorted rows (four 16-row mask tiles) and 64 output columns.
+// Its four waves OR the four tile masks into the same scalar union and split
+// the union's kernel offsets round-robin (wave w takes ordinals w, w+4, ...),
+// so the longest dependent chain of a block is a quarter of its offsets.  Each
+// wave accumulates all 64x64 outputs of its offsets in MFMA registers:
+// gathered rows and weight rows are loaded straight into operand registers
+// (16 B per lane) with raw buffer loads whose out-of-range offset reads zero,
+// used for rows without a neighbour (index -1) and tiles without the offset,
+// so no branch or select waits on the load stream.  Offsets are taken two at a
+// time (one memory round trip per pair) and the next pair's indices load while
+// the MFMAs run; tiles whose own mask lacks an offset skip its MFMAs.  At the
+// end the four partial tiles are summed through LDS in wave order (fixed,
+// deterministic) and each wave writes 16 rows with whole-row coalesced stores
+// through the epilogue.
+// Instantiated per input dtype in rg_<dtype>.hip.
 #pragma once
 #include "cs_device.h"
 #include "cs_kernels.h"
+#include "cs_rowgather_ws.h"
 
 namespace cs {
 
-template <typename T, typename OutT, int NB, int KCH>
+// Reduction chunks held per offset and pass.
+template <typename T> struct RgChunks { static constexpr int value = 2; };
+
+constexpr int kRgRows = 64;       // rows per block (4 mask tiles)
+constexpr int kRgRedPitch = 68;   // LDS floats per row of a partial tile
+
+// HAS_PAIRS: gather through a pair table (else the identity / a_inds-free GEMM).
+// FULL: kred is a multiple of the per-pass reduction width and nout a multiple
+// of 64, so no lane needs a reduction- or column-range predicate.
+template <typename T, typename OutT, bool HAS_PAIRS, bool FULL>
 __global__ __launch_bounds__(256) void rowgather_kernel(RowGatherArgs p) {
   using MM = Mma<T>;
   constexpr int E = MM::kElems;
   constexpr int CH = MM::kChunk;
+  constexpr int KCH = RgChunks<T>::value;
+  constexpr int NB = 4;
+  __shared__ float red[4][kRgRows][kRgRedPitch];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int row0 = (blockIdx.x * kWavesPerBlock + wave) * kRowsPerWave;
-  if (row0 >= p.m) return;  // whole wave leaves together
+  // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so give
+  // each XCD a contiguous range of row tiles (shared neighbours stay in one L2).
+  const int nblk = gridDim.x;
+  const int bid = blockIdx.x;
+  const int q = nblk >> 3, r8 = nblk & 7;
+  const int xcd = bid & 7;
+  const int slot = bid >> 3;
+  // XCDs below r8 get q + 1 consecutive tiles, the others q: a bijection on [0, nblk)
+  const int tile_blk = xcd < r8 ? xcd * (q + 1) + slot : r8 * (q + 1) + (xcd - r8) * q + slot;
+  const int row0 = tile_blk * kRgRows;
+  if (row0 >= p.m) return;  // whole block leaves together
   const int n0 = blockIdx.y * (NB * 16);
   const int lr = lane & 15;
   const int lg = lane >> 4;
@@ -44,7 +77,7 @@ __global__ __launch_bounds__(256) void rowgather_kernel(RowGatherArgs p) {
 #pragma unroll
   for (int s = 0; s < 4; ++s) sm[s] = __builtin_amdgcn_readlane(g, 16 * s) & p.mask_filter & kvmask;
 
-  if (p.mask_out != nullptr && lane < 4 && row0 + 16 * lane < p.m) {
+  if (wave == 0 && p.mask_out != nullptr && lane < 4 && row0 + 16 * lane < p.m) {
     const uint32_t v = lane == 0 ? sm[0] : lane == 1 ? sm[1] : lane == 2 ? sm[2] : sm[3];
     p.mask_out[row0 / kMaskTile + lane] = v;
   }
@@ -57,7 +90,17 @@ __global__ __launch_bounds__(256) void rowgather_kernel(RowGatherArgs p) {
 #pragma unroll
     for (int s = 0; s < 4; ++s) sm[s] = __builtin_bitreverse32(sm[s]) >> (32 - p.kv);
   }
-  uint32_t uni = (sm[0] | sm[1] | sm[2] | sm[3]);
+  const uint32_t uni = (sm[0] | sm[1] | sm[2] | sm[3]);
+  // this wave's share: union ordinals congruent to the wave index mod 4
+  uint32_t mine = 0;
+  {
+    uint32_t r = uni;
+    for (int o = 0; r; ++o) {
+      const uint32_t low = r & (0u - r);
+      if ((o & 3) == wave) mine |= low;
+      r ^= low;
+    }
+  }
 
   int cols[4];
 #pragma unroll
@@ -69,94 +112,137 @@ __global__ __launch_bounds__(256) void rowgather_kernel(RowGatherArgs p) {
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) acc[s][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-  const T* A = reinterpret_cast<const T*>(p.a);
-  while (uni) {
-    const int f = __builtin_ctz(uni);
-    uni &= uni - 1u;
-    int idx[4];
+  // pair indices of offset f for the rows this lane feeds (-1: no neighbour).
+  // The loaded value is not touched here, so the load stays in flight.
+  auto load_idx = [&](int f, int (&idx)[4]) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) idx[s] = HAS_PAIRS ? p.pairs[(int64_t)f * p.pair_ld + cols[s]] : cols[s];
+  };
+  // operand fragments of offset f (f < 0: none) for reduction block kb; rows
+  // with index -1, tiles without f, columns past nout and reduction elements
+  // past kred get an out-of-range offset and read zeros
+  const __amdgpu_buffer_rsrc_t ra = make_rsrc(p.a, p.a_bytes);
+  const __amdgpu_buffer_rsrc_t rb = make_rsrc(p.b, p.b_bytes);
+  auto load_frags = [&](int f, const int (&idx)[4], int kb, u32x4 (&af)[4][KCH], u32x4 (&bfr)[KCH][NB]) {
+    const bool fvalid = f >= 0;
+    const uint32_t fbit = fvalid ? (1u << f) : 0u;
+    const int64_t fb = (int64_t)(fvalid ? f : 0) * p.b_fs;
+#pragma unroll
+    for (int kc = 0; kc < KCH; ++kc) {
+      const int k = kb + kc * CH + lg * E;
+      const bool kin = fvalid && (FULL || k < p.kred);
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        const int n = n0 + nb * 16 + lr;
+        const uint32_t off = (kin && (FULL || n < p.nout))
+                                 ? (uint32_t)((fb + (int64_t)n * p.b_ns + k) * (int64_t)sizeof(T)) : kBufOOB;
+        bfr[kc][nb] = bld16(rb, off);
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const bool on = kin && (sm[s] & fbit) != 0u && idx[s] >= 0;
+        const uint32_t off = on ? (uint32_t)(((int64_t)idx[s] * p.lda + k) * (int64_t)sizeof(T)) : kBufOOB;
+        af[s][kc] = bld16(ra, off);
+      }
+    }
+  };
+  auto compute = [&](int f, const u32x4 (&af)[4][KCH], const u32x4 (&bfr)[KCH][NB]) {
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-      idx[s] = -1;
-      if ((sm[s] >> f) & 1u) idx[s] = p.pairs ? p.pairs[(int64_t)f * p.pair_ld + cols[s]] : cols[s];
-    }
-    const T* bf = reinterpret_cast<const T*>(p.b) + (int64_t)f * p.b_fs;
-    for (int kb = 0; kb < p.kred; kb += KCH * CH) {
+      if ((sm[s] >> f) & 1u) {
 #pragma unroll
-      for (int kc = 0; kc < KCH; ++kc) {
-        const int k = kb + kc * CH + lg * E;
-        const bool kin = k < p.kred;
-        u32x4 bfr[NB];
+        for (int kc = 0; kc < KCH; ++kc)
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-          const int n = n0 + nb * 16 + lr;
-          bfr[nb] = (kin && n < p.nout) ? ld16(bf + (int64_t)n * p.b_ns + k) : zero16();
-        }
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          if ((sm[s] >> f) & 1u) {
-            const u32x4 afr = (kin && idx[s] >= 0) ? ld16(A + (int64_t)idx[s] * p.lda + k) : zero16();
-#pragma unroll
-            for (int nb = 0; nb < NB; ++nb) MM::step(acc[s][nb], afr, bfr[nb]);
-          }
-        }
+          for (int nb = 0; nb < NB; ++nb) MM::step(acc[s][nb], af[s][kc], bfr[kc][nb]);
       }
+    }
+  };
+
+  // This wave's offsets, two at a time: a pair's fragment loads form one
+  // memory round trip, and the next pair's indices load while the MFMAs run.
+  for (int kb = 0; kb < p.kred; kb += KCH * CH) {
+    if (!mine) break;
+    uint32_t rem = mine;
+    auto pop = [&]() -> int {
+      const int f = rem ? __builtin_ctz(rem) : -1;
+      rem &= rem - 1u;
+      return f;
+    };
+    int fa = pop(), fb = pop();
+    int Ia[4], Ib[4];
+    load_idx(fa, Ia);
+    load_idx(fb < 0 ? fa : fb, Ib);
+    while (true) {
+      u32x4 Aa[4][KCH], Ba[KCH][NB], Ab[4][KCH], Bb[KCH][NB];
+      load_frags(fa, Ia, kb, Aa, Ba);
+      load_frags(fb, Ib, kb, Ab, Bb);
+      const int fc = pop(), fd = pop();
+      if (fc >= 0) {
+        load_idx(fc, Ia);
+        load_idx(fd < 0 ? fc : fd, Ib);
+      }
+      compute(fa, Aa, Ba);
+      if (fb >= 0) compute(fb, Ab, Bb);
+      if (fc < 0) break;
+      fa = fc;
+      fb = fd;
     }
   }
 
+  // partial tiles -> LDS; accumulator element i of block (s, nb) is row
+  // 16s + 4lg + i, column 16nb + lr
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) red[wave][16 * s + 4 * lg + i][16 * nb + lr] = acc[s][nb][i];
+  __syncthreads();
+
+  // wave w finishes tile w: fixed-order sum of the four partials, epilogue,
+  // one 64-column row per store instruction
+  const int s = wave;
+  if (!((live >> s) & 1u)) return;
   OutT* C = reinterpret_cast<OutT*>(p.c);
   const OutT* D = reinterpret_cast<const OutT*>(p.d);
-#pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    if (!((live >> s) & 1u)) continue;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int rr = 16 * s + 4 * lg + i;
-      const int orow = __shfl(myout, rr);
-      if (row0 + rr < p.m) {
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-          const int n = n0 + nb * 16 + lr;
-          if (n < p.nout) {
-            float v = acc[s][nb][i];
-            if (p.scale) v *= p.scale[n];
-            v *= p.alpha;
-            if (p.beta != 0.f) {
-              const float src = p.d_is_bias ? to_f32(D[n]) : to_f32(D[(int64_t)orow * p.ldd + n]);
-              v += p.beta * src;
-            }
-            v = apply_act(v, p.act_type, p.act_alpha);
-            C[(int64_t)orow * p.ldc + n] = from_f32<OutT>(v);
-          }
-        }
-      }
+  const int n = n0 + lane;
+  const bool ncol = n < p.nout;
+  const float scl = (p.scale && ncol) ? p.scale[n] : 1.f;
+  const float bias = (p.beta != 0.f && p.d_is_bias && ncol) ? to_f32(D[n]) : 0.f;
+#pragma unroll 4
+  for (int i = 0; i < 16; ++i) {
+    const int rr = 16 * s + i;
+    const int orow = __shfl(myout, rr);
+    if (row0 + rr < p.m && ncol) {
+      float v = ((red[0][rr][lane] + red[1][rr][lane]) + red[2][rr][lane]) + red[3][rr][lane];
+      v *= scl;
+      v *= p.alpha;
+      if (p.beta != 0.f) v += p.beta * (p.d_is_bias ? bias : to_f32(D[(int64_t)orow * p.ldd + n]));
+      v = apply_act(v, p.act_type, p.act_alpha);
+      C[(int64_t)orow * p.ldc + n] = from_f32<OutT>(v);
     }
   }
-}
-
-
-template <typename T, typename OutT, int NB>
-static hipError_t launch_rg_nb(const RowGatherArgs& a, hipStream_t s) {
-  constexpr int CH = Mma<T>::kChunk;
-  const int chunks = (a.kred + CH - 1) / CH;
-  dim3 grid((a.m + kRowsPerWave * kWavesPerBlock - 1) / (kRowsPerWave * kWavesPerBlock),
-            (a.nout + NB * 16 - 1) / (NB * 16), 1);
-  dim3 block(64 * kWavesPerBlock, 1, 1);
-  if (grid.x == 0 || grid.y == 0) return hipSuccess;
-  if (chunks <= 1)
-    hipLaunchKernelGGL((rowgather_kernel<T, OutT, NB, 1>), grid, block, 0, s, a);
-  else if (chunks <= 2)
-    hipLaunchKernelGGL((rowgather_kernel<T, OutT, NB, 2>), grid, block, 0, s, a);
-  else
-    hipLaunchKernelGGL((rowgather_kernel<T, OutT, NB, 4>), grid, block, 0, s, a);
-  return hipGetLastError();
 }
 
 template <typename T, typename OutT>
 static hipError_t launch_rg(const RowGatherArgs& a, int block_n, hipStream_t s) {
-  if (block_n == 64) return launch_rg_nb<T, OutT, 4>(a, s);
-  if (block_n == 128) return launch_rg_nb<T, OutT, 8>(a, s);
-  return hipErrorInvalidValue;
+  if (block_n != 64) return hipErrorInvalidValue;
+  bool used = false;
+  const hipError_t e = launch_ws<T, OutT>(a, s, &used);
+  if (used || e != hipSuccess) return e;
+  dim3 grid((a.m + kRgRows - 1) / kRgRows, (a.nout + 63) / 64, 1);
+  dim3 block(256, 1, 1);
+  if (grid.x == 0 || grid.y == 0) return hipSuccess;
+  constexpr int pass = RgChunks<T>::value * Mma<T>::kChunk;
+  const bool full = a.kred % pass == 0 && a.nout % 64 == 0;
+  if (a.pairs) {
+    if (full) hipLaunchKernelGGL((rowgather_kernel<T, OutT, true, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((rowgather_kernel<T, OutT, true, false>), grid, block, 0, s, a);
+  } else {
+    if (full) hipLaunchKernelGGL((rowgather_kernel<T, OutT, false, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((rowgather_kernel<T, OutT, false, false>), grid, block, 0, s, a);
+  }
+  return hipGetLastError();
 }
-
 
 }  // namespace cs

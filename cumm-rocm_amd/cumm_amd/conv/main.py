@@ -70,11 +70,9 @@ class ConvMainUnitTest:
 def select_conv_algo(op_type: int, dtype_input: int, dtype_weight: int, dtype_output: int,
                      C: int, K: int) -> ConvAlgoDesp:
     """Pick this build's kernel for a conv layer, the way a caller filters
-    ``get_all_conv_algo_desp()`` by op and dtypes: 64 output columns per wave
-    when the written tensor has <= 64 channels, else 128."""
+    ``get_all_conv_algo_desp()`` by op and dtypes (64 output columns per wave)."""
     op = int(op_type)
-    nout = {0: K, 1: C, 2: K}[op]
-    bn = 64 if nout <= 64 else 128
+    bn = 64
     best = None
     for d in ConvMainUnitTest.get_all_conv_algo_desp():
         if int(d.op_type) != op:

@@ -164,3 +164,37 @@ def test_ragged_and_tiny():
         exp, tm = ref.conv_forward(c.X, c.W, c.prob.pair_fwd, c.prob.mask_sorted, c.prob.argsort, 16)
         assert np.array_equal(mo, tm)
         assert _rel(out, exp) <= 5e-6
+
+
+def _e4m3_quant(x, axis_scale):
+    """OCP E4M3 quantisation with absmax scales (448 = largest finite e4m3)."""
+    q = torch.from_numpy((x / axis_scale).astype(np.float32)).to(torch.float8_e4m3fn)
+    return q, q.to(torch.float64).numpy()
+
+
+@pytest.mark.parametrize("out_dt", ["f16", "bf16"])
+def test_forward_fp8_per_channel_scales(case64, out_dt):
+    """FP8-E4M3 features (per-tensor scale) and weights (per-output-channel
+    absmax scales), FP32 accumulation, scale[k] = s_x * s_w[k] applied in the
+    epilogue.  Parity vs the oracle on the dequantised operands; the error
+    against the FP64 result on the unquantised inputs is reported, not gated."""
+    c = case64
+    prob = c.prob
+    s_x = np.abs(c.X).max() / 448.0
+    s_w = np.abs(c.W).reshape(c.K, -1).max(axis=1) / 448.0
+    xq, xq64 = _e4m3_quant(c.X, s_x)
+    wq, wq64 = _e4m3_quant(c.W, s_w[:, None, None])
+    scale = (s_x * s_w).astype(np.float32)
+    desp = desp_for(0, "e4m3", c.C, c.K, out_dt)
+    out = torch.zeros((prob.n, c.K), dtype=TORCH_DT[out_dt], device="cuda")
+    p = conv_params(0, desp, xq.cuda(), wq.cuda(), out, prob.pair_fwd, prob.mask_sorted, prob.argsort,
+                    scale=torch.from_numpy(scale).cuda())
+    ConvMainUnitTest.implicit_gemm2(p)
+    torch.cuda.synchronize()
+    got = out.double().cpu().numpy()
+    exp, _ = ref.conv_forward(xq64, wq64, prob.pair_fwd, prob.mask_sorted, prob.argsort, 16, scale=scale)
+    assert _rel(got, quantize(exp, out_dt)) <= TOL[out_dt]
+    full, _ = ref.conv_forward(c.X, c.W, prob.pair_fwd, prob.mask_sorted, prob.argsort, 16)
+    mx, rel = ref.relative_error(got, full)
+    print(f"[fp8] out={out_dt} error vs FP64 on unquantised inputs: rel={rel:.3e} max_abs={mx:.3e}")
+    assert rel < 0.1
