@@ -429,10 +429,12 @@ int64_t align256(int64_t v) { return (v + 255) / 256 * 256; }
 // when > 1, else a fixed default; depends only on the call's parameters.
 int resolve_splits(int requested) {
   if (requested > 1) return std::min(requested, cs::kWgradMaxBlocks);
-  return 32;
+  return cs::kWgradMaxBlocks;
 }
 
-constexpr int kWgradMinGroups = 8;
+// Lower bound of 16-row groups per wgrad piece: pieces of an offset scale with
+// its active groups, at least 2 chunks of 256 rows each.
+constexpr int kWgradMinGroups = 32;
 
 struct WgradLayout {
   int64_t partial = 0, glist = 0, gcount = 0, total = 0;
@@ -640,6 +642,8 @@ void run_conv(const cs_conv_params& p) {
     w.lda = row_stride(p.output);
     w.b = p.input.data;
     w.ldb = row_stride(p.input);
+    w.a_bytes = (uint32_t)span_bytes(p.output);
+    w.b_bytes = (uint32_t)span_bytes(p.input);
     w.a_rows = reinterpret_cast<const int32_t*>(p.mask_argsort.data);
     w.pairs = reinterpret_cast<const int32_t*>(p.indices.data);
     w.pair_ld = p.indices.stride[0];
@@ -805,6 +809,8 @@ void run_gemm(const cs_gemm_params& p) {
     w.lda = row_stride(p.a);
     w.b = p.b.data;
     w.ldb = row_stride(p.b);
+    w.a_bytes = (uint32_t)span_bytes(p.a);
+    w.b_bytes = (uint32_t)span_bytes(p.b);
     w.a_rows = reinterpret_cast<const int32_t*>(p.a_inds.data);
     w.b_rows = reinterpret_cast<const int32_t*>(p.b_inds.data);
     w.pairs = nullptr;

@@ -41,7 +41,7 @@ struct RowGatherArgs {
 
 // Row-pair outer-product reduction (sparse conv wgrad, ShuffleAB GEMM).
 //   out[f][mm][nn] = sum_r A[ra(r)][mm] * B[rb(r,f)][nn]
-// ra(r) = a_rows ? a_rows[r] : r;  rb(r,f) = pairs ? pairs[f*pair_ld + ra(r)]
+// ra(r) = a_rows[r];  rb(r,f) = pairs ? pairs[f*pair_ld + ra(r)]
 // : (b_rows ? b_rows[r] : r); row r counts for offset f only if rb >= 0 and
 // (mask == null or bit f of mask[r / mask_width]).
 // A plan kernel lists, per offset, the 16-row groups whose mask word has the
@@ -52,14 +52,15 @@ struct RowGatherArgs {
 struct WgradArgs {
   const void* a; int64_t lda;
   const void* b; int64_t ldb;
-  const int32_t* a_rows;
-  const int32_t* b_rows;
-  const int32_t* pairs; int64_t pair_ld;
+  uint32_t a_bytes, b_bytes;  // extents of a and b (< 2^31): buffer-load range
+  const int32_t* a_rows;      // required
+  const int32_t* b_rows;      // required when pairs is null
+  const int32_t* pairs; int64_t pair_ld;  // pairs rows hold nrows entries
   const uint32_t* mask; int32_t mask_width; int32_t num_mask;
   int32_t nrows, M, N, kv;
   uint32_t mask_filter;
   int32_t ngroups;           // div_up(nrows, 16)
-  int32_t max_blocks;        // pieces per offset (grid.z), <= 64
+  int32_t max_blocks;        // pieces per offset (grid.z), <= kWgradMaxBlocks
   int32_t min_groups;        // groups per piece, lower bound
   int32_t* glist;            // [kv][ngroups]
   int32_t* gcount;           // [kv]
@@ -99,7 +100,7 @@ constexpr int kRowsPerWave = 64;     // four 16-row mask tiles per wave
 constexpr int kWavesPerBlock = 4;
 constexpr int kMaskTile = 16;        // rows per kernel-offset mask tile
 constexpr int kWgradRows = 64;       // rows staged per wgrad step
-constexpr int kWgradMaxBlocks = 64;  // pieces per offset, upper bound
+constexpr int kWgradMaxBlocks = 128; // pieces per offset, upper bound
 constexpr int kWgradTile = 64;       // wgrad output tile (M and N)
 
 }  // namespace cs

@@ -49,35 +49,66 @@ __device__ __forceinline__ int wgrad_pieces(int count, int max_blocks, int min_g
 }
 
 // One block per offset: list the 16-row groups whose reduced mask word has the
-// offset, in row order.
+// offset, in row order.  Each thread reads PG consecutive mask words at once
+// (one round trip per 256*PG groups), then a block-wide exclusive scan of the
+// per-thread counts places its groups.
 __global__ __launch_bounds__(256) void wgrad_plan_kernel(WgradArgs p) {
+  constexpr int PG = 32;
   const int f = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  __shared__ int wave_cnt[4];
+  __shared__ int wave_sum[4];
   if (!((p.mask_filter >> f) & 1u)) {
     if (tid == 0) p.gcount[f] = 0;
     return;
   }
+  const __amdgpu_buffer_rsrc_t mrs = make_rsrc(p.mask, p.mask ? (uint32_t)p.num_mask * 4u : 0u);
   int base = 0;
-  for (int c = 0; c < p.ngroups; c += 256) {
-    const int g = c + tid;
-    bool act = g < p.ngroups;
-    if (act && p.mask != nullptr) {
-      const int mi = (g * 16) / p.mask_width;
-      act = mi < p.num_mask && ((p.mask[mi] >> f) & 1u);
+  for (int c0 = 0; c0 < p.ngroups; c0 += 256 * PG) {
+    const int g0 = c0 + tid * PG;
+    // unconditional buffer loads (out-of-range words read as 0), all in flight at once
+    uint32_t w[PG];
+    if (p.mask) {
+#pragma unroll
+      for (int i = 0; i < PG; ++i) {
+        const uint32_t wi = (uint32_t)(g0 + i) * 16u / (uint32_t)p.mask_width;
+        w[i] = __builtin_amdgcn_raw_buffer_load_b32(mrs, g0 + i < p.ngroups ? wi * 4u : kBufOOB, 0, 0);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < PG; ++i) w[i] = g0 + i < p.ngroups ? 0xffffffffu : 0u;
     }
-    const uint64_t bal = __ballot(act);
-    const int pre = __popcll(bal & ((1ull << lane) - 1ull));
-    if (lane == 0) wave_cnt[wave] = __popcll(bal);
+    uint32_t bits = 0u;
+#pragma unroll
+    for (int i = 0; i < PG; ++i) bits |= ((w[i] >> f) & 1u) << i;
+    const int cnt = __popc(bits);
+    // inclusive scan of cnt over the wave
+    int incl = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int v = __shfl_up(incl, d);
+      if (lane >= d) incl += v;
+    }
+    if (lane == 63) wave_sum[wave] = incl;
     __syncthreads();
-    int off = base;
-    for (int w = 0; w < wave; ++w) off += wave_cnt[w];
-    if (act) p.glist[(int64_t)f * p.ngroups + off + pre] = g;
-    base += wave_cnt[0] + wave_cnt[1] + wave_cnt[2] + wave_cnt[3];
+    int off = base + incl - cnt;
+    for (int wv = 0; wv < wave; ++wv) off += wave_sum[wv];
+    int* out = p.glist + (int64_t)f * p.ngroups + off;
+    int k = 0;
+    while (bits) {
+      const int i = __builtin_ctz(bits);
+      bits &= bits - 1u;
+      out[k++] = g0 + i;
+    }
+    base += wave_sum[0] + wave_sum[1] + wave_sum[2] + wave_sum[3];
     __syncthreads();
   }
   if (tid == 0) p.gcount[f] = base;
 }
+
+// Rows staged per thread and chunk: 4 (256-row chunks) for 1- and 2-byte types,
+// 2 for f32, whose 16-byte loads cover half as many channels.
+template <typename T> struct WgRows { static constexpr int value = 4; };
+template <> struct WgRows<float> { static constexpr int value = 2; };
 
 template <typename T>
 __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs p) {
@@ -88,16 +119,18 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs p) {
   constexpr int TS = kWgradTile;
   constexpr int PITCH = R * (int)sizeof(T) + 16;
   constexpr int CHUNKS = TS / E;          // 16-byte chunks per 64 channels
-  constexpr int CPT = (CHUNKS + 3) / 4;   // chunks per thread per operand
+  constexpr int CPT = (CHUNKS + 3) / 4;   // chunks per thread per operand and row
+  constexpr int RPT = WgRows<T>::value;   // rows per thread per chunk
+  constexpr int CROWS = RPT * R;          // rows per chunk
   __shared__ __attribute__((aligned(16))) char lds_a[TS * PITCH];
   __shared__ __attribute__((aligned(16))) char lds_b[TS * PITCH];
-  __shared__ int s_ra[256], s_rb[256];
 
   const int f = blockIdx.y;
   if (!((p.mask_filter >> f) & 1u)) return;
   const int count = p.gcount[f];
   const int nb = wgrad_pieces(count, p.max_blocks, p.min_groups);
   const int piece = blockIdx.z;
+  if (piece >= nb) return;
   const int tiles_n = (p.N + TS - 1) / TS;
   const int m0 = (blockIdx.x / tiles_n) * TS;
   const int nn0 = (blockIdx.x % tiles_n) * TS;
@@ -106,12 +139,12 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs p) {
   const int lr = lane & 15, lg = lane >> 4;
   const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
   const int srow = tid & 63, sq = tid >> 6;
-  if (piece >= nb) return;
   const int gpb = (count + nb - 1) / nb;
   const int g0 = piece * gpb;
   const int g1 = min(g0 + gpb, count);
   const int nrows_blk = (g1 - g0) * 16;
   const int* glist = p.glist + (int64_t)f * p.ngroups + g0;
+  const int nchunks = (nrows_blk + CROWS - 1) / CROWS;
 
   f32x4 acc[2][2];
 #pragma unroll
@@ -119,62 +152,97 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs p) {
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-  const T* A = reinterpret_cast<const T*>(p.a);
-  const T* B = reinterpret_cast<const T*>(p.b);
+  // Every index and data load is an unconditional buffer load: invalid lanes
+  // use an out-of-range offset and read zeros, so no load sits behind a branch
+  // and all of a chunk's loads are in flight together.
+  const __amdgpu_buffer_rsrc_t rA = make_rsrc(p.a, p.a_bytes);
+  const __amdgpu_buffer_rsrc_t rB = make_rsrc(p.b, p.b_bytes);
+  const __amdgpu_buffer_rsrc_t rG = make_rsrc(glist, (uint32_t)(g1 - g0) * 4u);
+  const __amdgpu_buffer_rsrc_t rAr = make_rsrc(p.a_rows, (uint32_t)p.nrows * 4u);
+  const __amdgpu_buffer_rsrc_t rBr =
+      p.pairs ? make_rsrc(p.pairs + (int64_t)f * p.pair_ld, (uint32_t)p.nrows * 4u)
+              : make_rsrc(p.b_rows, (uint32_t)p.nrows * 4u);
+  const bool by_pairs = p.pairs != nullptr;
 
-  // rows of super-stage j (256 list positions): thread t resolves position j*256+t
-  auto fetch_rows = [&](int j, int& ra, int& rb) {
-    const int pos = j * 256 + tid;
-    ra = -1;
-    rb = -1;
-    if (pos < nrows_blk) {
-      const int r = glist[pos >> 4] * 16 + (pos & 15);
-      if (r < p.nrows) {
-        ra = p.a_rows ? p.a_rows[r] : r;
-        rb = p.pairs ? p.pairs[(int64_t)f * p.pair_ld + ra] : (p.b_rows ? p.b_rows[r] : r);
+  // Four-stage index pipeline per staged row: in the iteration of chunk j the
+  // thread loads the data rows of chunk j, the b-row index of chunk j+1, the
+  // a-row (argsort) index of chunk j+2 and the group id of chunk j+3, so each
+  // chunk waits for one round trip.  A value loaded in one iteration is first
+  // read in the next one.
+  auto posof = [&](int j, int st) { return j * CROWS + st * R + srow; };
+  auto ld_group = [&](int j, int st) -> int {
+    const int pos = posof(j, st);
+    return (int)__builtin_amdgcn_raw_buffer_load_b32(rG, pos < nrows_blk ? (uint32_t)(pos >> 4) * 4u : kBufOOB, 0, 0);
+  };
+  int graw2[RPT], araw1[RPT], r1[RPT], brw0[RPT], aeff0[RPT];
+  bool v1[RPT];
+  {
+    int g0r[RPT], g1r[RPT];
+#pragma unroll
+    for (int st = 0; st < RPT; ++st) {
+      g0r[st] = ld_group(0, st);
+      g1r[st] = ld_group(1, st);
+      graw2[st] = ld_group(2, st);
+    }
+    int r0[RPT], araw0[RPT];
+    bool v0[RPT];
+#pragma unroll
+    for (int st = 0; st < RPT; ++st) {
+      const int p0 = posof(0, st), p1 = posof(1, st);
+      r0[st] = g0r[st] * 16 + (p0 & 15);
+      r1[st] = g1r[st] * 16 + (p1 & 15);
+      v0[st] = p0 < nrows_blk && r0[st] < p.nrows;
+      v1[st] = p1 < nrows_blk && r1[st] < p.nrows;
+      araw0[st] = (int)__builtin_amdgcn_raw_buffer_load_b32(rAr, v0[st] ? (uint32_t)r0[st] * 4u : kBufOOB, 0, 0);
+      araw1[st] = (int)__builtin_amdgcn_raw_buffer_load_b32(rAr, v1[st] ? (uint32_t)r1[st] * 4u : kBufOOB, 0, 0);
+    }
+#pragma unroll
+    for (int st = 0; st < RPT; ++st) {
+      aeff0[st] = v0[st] ? araw0[st] : -1;
+      const uint32_t off = by_pairs ? (uint32_t)aeff0[st] * 4u : (uint32_t)r0[st] * 4u;
+      brw0[st] = (int)__builtin_amdgcn_raw_buffer_load_b32(rBr, aeff0[st] >= 0 ? off : kBufOOB, 0, 0);
+    }
+  }
+
+  for (int j = 0; j < nchunks; ++j) {
+    u32x4 va[RPT][CPT], vb[RPT][CPT];
+#pragma unroll
+    for (int st = 0; st < RPT; ++st) {
+      const int rb = aeff0[st] >= 0 ? brw0[st] : -1;
+      const bool ok = rb >= 0;
+#pragma unroll
+      for (int i = 0; i < CPT; ++i) {
+        const int ch = (sq + 4 * i) * E;
+        const bool oka = ok && ch < TS && m0 + ch < p.M;
+        const bool okb = ok && ch < TS && nn0 + ch < p.N;
+        va[st][i] = bld16(rA, oka ? (uint32_t)(((int64_t)aeff0[st] * p.lda + m0 + ch) * (int)sizeof(T)) : kBufOOB);
+        vb[st][i] = bld16(rB, okb ? (uint32_t)(((int64_t)rb * p.ldb + nn0 + ch) * (int)sizeof(T)) : kBufOOB);
       }
     }
-  };
-  // 16-byte chunks of stage st (64 rows of the current super-stage)
-  auto load_data = [&](int st, u32x4 (&va)[CPT], u32x4 (&vb)[CPT]) {
-    const int ra = s_ra[st * R + srow];
-    const int rb = s_rb[st * R + srow];
-    const bool ok = rb >= 0;
+    int brw1[RPT], aeff1[RPT], araw2[RPT], r2[RPT], graw3[RPT];
+    bool v2[RPT];
 #pragma unroll
-    for (int i = 0; i < CPT; ++i) {
-      const int ch = (sq + 4 * i) * E;
-      const bool oka = ok && ch < TS && m0 + ch < p.M;
-      const bool okb = ok && ch < TS && nn0 + ch < p.N;
-      const u32x4 x = ld16(oka ? A + (int64_t)ra * p.lda + m0 + ch : A);
-      const u32x4 y = ld16(okb ? B + (int64_t)rb * p.ldb + nn0 + ch : B);
-      va[i] = oka ? x : zero16();
-      vb[i] = okb ? y : zero16();
+    for (int st = 0; st < RPT; ++st) {
+      aeff1[st] = v1[st] ? araw1[st] : -1;
+      const uint32_t off = by_pairs ? (uint32_t)aeff1[st] * 4u : (uint32_t)r1[st] * 4u;
+      brw1[st] = (int)__builtin_amdgcn_raw_buffer_load_b32(rBr, aeff1[st] >= 0 ? off : kBufOOB, 0, 0);
+      const int p2 = posof(j + 2, st);
+      r2[st] = graw2[st] * 16 + (p2 & 15);
+      v2[st] = p2 < nrows_blk && r2[st] < p.nrows;
+      araw2[st] = (int)__builtin_amdgcn_raw_buffer_load_b32(rAr, v2[st] ? (uint32_t)r2[st] * 4u : kBufOOB, 0, 0);
+      graw3[st] = ld_group(j + 3, st);
     }
-  };
-
-  const int nsuper = (nrows_blk + 255) / 256;
-  int cra, crb;
-  fetch_rows(0, cra, crb);
-  for (int j = 0; j < nsuper; ++j) {
-    __syncthreads();  // previous super-stage done with s_ra / s_rb
-    s_ra[tid] = cra;
-    s_rb[tid] = crb;
-    if (j + 1 < nsuper) fetch_rows(j + 1, cra, crb);
-    __syncthreads();
-    const int rows_left = nrows_blk - j * 256;
-    const int nst = min(4, (rows_left + R - 1) / R);
-    u32x4 va[CPT], vb[CPT];
-    load_data(0, va, vb);
-    for (int st = 0; st < nst; ++st) {
+#pragma unroll
+    for (int st = 0; st < RPT; ++st) {
+      __syncthreads();  // previous stage's MFMAs are done with the tiles
 #pragma unroll
       for (int i = 0; i < CPT; ++i) {
         const int ch = (sq + 4 * i) * E;
         if (ch < TS) {
-          TrStore<T>::put(lds_a, PITCH, ch, srow, va[i]);
-          TrStore<T>::put(lds_b, PITCH, ch, srow, vb[i]);
+          TrStore<T>::put(lds_a, PITCH, ch, srow, va[st][i]);
+          TrStore<T>::put(lds_b, PITCH, ch, srow, vb[st][i]);
         }
       }
-      if (st + 1 < nst) load_data(st + 1, va, vb);
       __syncthreads();
 #pragma unroll
       for (int kc = 0; kc < R; kc += CH) {
@@ -190,7 +258,15 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs p) {
 #pragma unroll
           for (int jj = 0; jj < 2; ++jj) MM::step(acc[i][jj], af[i], bfr[jj]);
       }
-      __syncthreads();
+    }
+#pragma unroll
+    for (int st = 0; st < RPT; ++st) {
+      aeff0[st] = aeff1[st];
+      brw0[st] = brw1[st];
+      araw1[st] = araw2[st];
+      r1[st] = r2[st];
+      v1[st] = v2[st];
+      graw2[st] = graw3[st];
     }
   }
 
@@ -210,7 +286,8 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs p) {
 // ============================================================ split-K tree reduce
 template <typename OutT>
 __global__ __launch_bounds__(256) void splitk_reduce_kernel(ReduceArgs p) {
-  constexpr int MAXS = kWgradMaxBlocks;
+  constexpr int L1 = 32;                     // first tree level
+  constexpr int L2 = kWgradMaxBlocks / L1;   // second tree level
   const int64_t mn = (int64_t)p.M * p.N;
   const int64_t total = (int64_t)p.kv * mn;
   OutT* out = reinterpret_cast<OutT*>(p.out);
@@ -223,15 +300,27 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(ReduceArgs p) {
     const int nn = (int)(rem - (int64_t)mm * p.N);
     const int nb = wgrad_pieces(p.gcount[f], p.max_blocks, p.min_groups);
     const float* src = p.partial + (int64_t)f * p.max_blocks * mn + rem;
-    float v[MAXS];
+    // fixed two-level pairwise tree over the pieces, independent of scheduling
+    float s2[L2];
 #pragma unroll
-    for (int s = 0; s < MAXS; ++s) v[s] = s < nb ? src[(int64_t)s * mn] : 0.f;
-    // fixed pairwise tree: ((v0+v1)+(v2+v3))+... independent of scheduling
+    for (int c = 0; c < L2; ++c) {
+      float v[L1];
 #pragma unroll
-    for (int w = MAXS / 2; w >= 1; w >>= 1)
+      for (int s = 0; s < L1; ++s) {
+        const int b = c * L1 + s;
+        v[s] = b < nb ? src[(int64_t)b * mn] : 0.f;
+      }
 #pragma unroll
-      for (int i = 0; i < w; ++i) v[i] = v[2 * i] + v[2 * i + 1];
-    float r = p.alpha * v[0];
+      for (int w = L1 / 2; w >= 1; w >>= 1)
+#pragma unroll
+        for (int i = 0; i < w; ++i) v[i] = v[2 * i] + v[2 * i + 1];
+      s2[c] = v[0];
+    }
+#pragma unroll
+    for (int w = L2 / 2; w >= 1; w >>= 1)
+#pragma unroll
+      for (int i = 0; i < w; ++i) s2[i] = s2[2 * i] + s2[2 * i + 1];
+    float r = p.alpha * s2[0];
     const int64_t oi = (int64_t)mm * p.o_ms + (int64_t)f * p.o_fs + nn;
     if (p.beta != 0.f) {
       const OutT* sp = reinterpret_cast<const OutT*>(p.src);

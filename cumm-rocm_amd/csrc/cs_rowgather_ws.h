@@ -4,7 +4,7 @@
 // ~1.5-2 us, and a kernel whose waves walk a tile's kernel offsets one round
 // trip at a time lasts as long as its heaviest tiles (up to 24 offsets).  This
 // kernel bounds every tile to three round trips (masks -> pair indices of up
-// to 8 offsets -> their gathered rows) and keeps register use low so 16 waves
+// to 8 offsets -> their gathered rows) and keeps register use low so 12 waves
 // fit on a CU:
 //   * persistent blocks, one per CU: the weights of all kv offsets for a slice
 //     of NCOL output columns are loaded once into LDS (<= 108 KiB, 16-byte
@@ -22,11 +22,12 @@
 namespace cs {
 
 constexpr int kWsLdsBytes = 110592;  // 108 KiB of weights per block
-constexpr int kWsWaves = 16;         // waves per block
+constexpr int kWsWaves = 12;         // waves per block (3 per SIMD: 168 VGPRs each)
+constexpr int kWsStageBytes = 2048;  // per-wave LDS tile for coalesced row stores
 
 // KC: reduction chunks (C / chunk); NBC: 16-column blocks per wave (NCOL / 16)
 template <typename T, typename OutT, int KC, int NBC>
-__global__ __launch_bounds__(1024) void rowgather_ws_kernel(RowGatherArgs p, int ntiles) {
+__global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherArgs p, int ntiles) {
   using MM = Mma<T>;
   constexpr int E = MM::kElems;
   constexpr int CH = MM::kChunk;
@@ -36,6 +37,7 @@ __global__ __launch_bounds__(1024) void rowgather_ws_kernel(RowGatherArgs p, int
   constexpr int SWZ = CHUNKS >= 8 ? 7 : CHUNKS - 1;
   constexpr int BATCH = KC > 2 ? 4 : 8;  // offsets per gather round trip
   __shared__ __attribute__((aligned(16))) char wlds[kWsLdsBytes];
+  __shared__ __attribute__((aligned(16))) char stage_lds[kWsWaves * kWsStageBytes];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -43,18 +45,32 @@ __global__ __launch_bounds__(1024) void rowgather_ws_kernel(RowGatherArgs p, int
   const int lr = lane & 15, lg = lane >> 4;
   const int n0 = blockIdx.y * NCOL;
 
-  // ---- weights of all offsets for columns [n0, n0 + NCOL) -> LDS
+  // ---- weights of all offsets for columns [n0, n0 + NCOL) -> LDS.  Each
+  // thread issues its (at most WPT) loads before writing any of them, so the
+  // preload costs one round trip instead of one per chunk.
   {
     const T* B = reinterpret_cast<const T*>(p.b);
     const int total = p.kv * NCOL * CHUNKS;
-    for (int i = tid; i < total; i += blockDim.x) {
+    constexpr int WPT = (kWsLdsBytes / 16 + 64 * kWsWaves - 1) / (64 * kWsWaves);
+    u32x4 v[WPT];
+#pragma unroll
+    for (int j = 0; j < WPT; ++j) {
+      const int i = tid + j * (64 * kWsWaves);
       const int c = i % CHUNKS;
       const int n = (i / CHUNKS) % NCOL;
       const int f = i / (CHUNKS * NCOL);
-      u32x4 v = zero16();
-      if (n0 + n < p.nout) v = ld16(B + (int64_t)f * p.b_fs + (int64_t)(n0 + n) * p.b_ns + c * E);
-      const int slot = ((f * NCOL + n) * CHUNKS + (c ^ (n & SWZ))) * 16;
-      *reinterpret_cast<u32x4*>(wlds + slot) = v;
+      const bool ok = i < total && n0 + n < p.nout;
+      v[j] = ok ? ld16(B + (int64_t)f * p.b_fs + (int64_t)(n0 + n) * p.b_ns + c * E) : zero16();
+    }
+#pragma unroll
+    for (int j = 0; j < WPT; ++j) {
+      const int i = tid + j * (64 * kWsWaves);
+      if (i < total) {
+        const int c = i % CHUNKS;
+        const int n = (i / CHUNKS) % NCOL;
+        const int f = i / (CHUNKS * NCOL);
+        *reinterpret_cast<u32x4*>(wlds + ((f * NCOL + n) * CHUNKS + (c ^ (n & SWZ))) * 16) = v[j];
+      }
     }
   }
   __syncthreads();
@@ -63,88 +79,143 @@ __global__ __launch_bounds__(1024) void rowgather_ws_kernel(RowGatherArgs p, int
   const uint32_t kvmask = p.kv >= 32 ? 0xffffffffu : ((1u << p.kv) - 1u);
   OutT* Cout = reinterpret_cast<OutT*>(p.c);
   const OutT* D = reinterpret_cast<const OutT*>(p.d);
-  const int gwaves = gridDim.x * kWsWaves;
+  const int G = gridDim.x * kWsWaves;
+  // coalesced row stores through a per-wave LDS tile (beta == 0, whole slice
+  // inside nout, 16-byte aligned rows)
+  constexpr int ROWB = NCOL * (int)sizeof(OutT);
+  constexpr bool kStage = ROWB % 16 == 0 && 16 * ROWB <= kWsStageBytes;
+  const bool vec_store = kStage && p.beta == 0.f && n0 + NCOL <= p.nout &&
+                         ((int64_t)p.ldc * (int64_t)sizeof(OutT)) % 16 == 0 &&
+                         ((reinterpret_cast<uintptr_t>(Cout) + (uintptr_t)n0 * sizeof(OutT)) & 15u) == 0;
+  char* stage = stage_lds + wave * kWsStageBytes;
 
-  for (int t = blockIdx.x * kWsWaves + wave; t < ntiles; t += gwaves) {
-    const int row0 = t * kMaskTile;
-    const int r = row0 + lr;
-    const bool rv = r < p.m;
-    uint32_t m = 0;
-    int col = 0, orow = 0;
-    if (rv) {
-      m = p.mask ? p.mask[r] : kvmask;
-      col = p.pair_col ? p.pair_col[r] : r;
-      orow = p.out_row ? p.out_row[r] : r;
+  // per-task row data: row mask, pair column (argsort) and output row of the
+  // lane's row (lr); rows past m read as empty
+  struct Meta { uint32_t m; int col, orow; };
+  auto load_meta = [&](int t, Meta& M) {
+    const int r = t * kMaskTile + lr;
+    M.m = 0u;
+    M.col = 0;
+    M.orow = 0;
+    if (t < ntiles && r < p.m) {
+      M.m = p.mask ? p.mask[r] : kvmask;
+      M.col = p.pair_col ? p.pair_col[r] : r;
+      M.orow = p.out_row ? p.out_row[r] : r;
     }
+  };
+  // tile mask (also stored to mask_output), filter, reverse
+  auto tile_mask = [&](int t, uint32_t m) -> uint32_t {
     m |= __shfl_xor(m, 1);
     m |= __shfl_xor(m, 2);
     m |= __shfl_xor(m, 4);
     m |= __shfl_xor(m, 8);
     uint32_t km = __builtin_amdgcn_readfirstlane(m) & p.mask_filter & kvmask;
     if (blockIdx.y == 0 && p.mask_out != nullptr && lane == 0) p.mask_out[t] = km;
-    if (p.skip_empty && km == 0u) continue;
-    if (p.reverse_mask) km = __builtin_bitreverse32(km) >> (32 - p.kv);
-
-    f32x4 acc[NBC];
+    return km;
+  };
+  auto prepare = [&](uint32_t km, bool& live) -> uint32_t {
+    live = !(p.skip_empty && km == 0u);
+    if (!live) return 0u;
+    return p.reverse_mask ? __builtin_bitreverse32(km) >> (32 - p.kv) : km;
+  };
+  // i-th set bit of a batch mask (-1 past the end); i is a compile-time slot
+  auto slot_f = [](uint32_t bm, int i) -> int {
+    for (int j = 0; j < i; ++j) bm &= bm - 1u;
+    return bm ? __builtin_ctz(bm) : -1;
+  };
+  // next batch: the first BATCH offsets of rem (returned as a bit mask);
+  // pair indices issued (lane group lg loads slots lg and lg + 4), not waited on
+  auto batch = [&](uint32_t& rem, int col, int& ia, int& ib) -> uint32_t {
+    uint32_t bm = 0u;
 #pragma unroll
-    for (int nb = 0; nb < NBC; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
-
-    uint32_t rem = km;
-    while (rem) {
-      // this batch's offsets (scalar), missing ones marked -1
-      int fo[BATCH];
+    for (int i = 0; i < BATCH; ++i) {
+      const uint32_t low = rem & (0u - rem);
+      bm |= low;
+      rem ^= low;
+    }
+    int fs[BATCH];
 #pragma unroll
-      for (int i = 0; i < BATCH; ++i) {
-        fo[i] = rem ? __builtin_ctz(rem) : -1;
-        rem &= rem - 1u;
+    for (int i = 0; i < BATCH; ++i) fs[i] = slot_f(bm, i);
+    const int fa = lg == 0 ? fs[0] : lg == 1 ? fs[1] : lg == 2 ? fs[2] : fs[3];
+    int fb = -1;
+    if constexpr (BATCH == 8) fb = lg == 0 ? fs[4] : lg == 1 ? fs[5] : lg == 2 ? fs[6] : fs[7];
+    if (p.pairs) {
+      ia = p.pairs[(int64_t)(fa < 0 ? 0 : fa) * p.pair_ld + col];
+      ib = -1;
+      if constexpr (BATCH == 8) ib = p.pairs[(int64_t)(fb < 0 ? 0 : fb) * p.pair_ld + col];
+    } else {
+      ia = col;
+      ib = col;
+    }
+    return bm;
+  };
+  // gathered rows of a batch: slot i's index for row lr sits in lane (i % 4) * 16 + lr
+  auto gather = [&](uint32_t bm, int ia, int ib, u32x4 (&af)[BATCH][KC]) {
+    const int cnt = __builtin_popcount(bm);
+#pragma unroll
+    for (int i = 0; i < BATCH; ++i) {
+      int idx = __shfl(i < 4 ? ia : ib, (i & 3) * 16 + lr);
+      idx = i < cnt ? idx : -1;
+#pragma unroll
+      for (int kc = 0; kc < KC; ++kc) {
+        const int k = kc * CH + lg * E;
+        const uint32_t off = idx >= 0 ? (uint32_t)(((int64_t)idx * p.lda + k) * (int64_t)sizeof(T)) : kBufOOB;
+        af[i][kc] = bld16(ra, off);
       }
-      // lane group lg loads the pair indices of batch slots lg and lg + 4
-      int fa = lg == 0 ? fo[0] : lg == 1 ? fo[1] : lg == 2 ? fo[2] : fo[3];
-      int fb = -1;
-      if constexpr (BATCH == 8) fb = lg == 0 ? fo[4] : lg == 1 ? fo[5] : lg == 2 ? fo[6] : fo[7];
-      int ia = -1, ib = -1;
-      if (p.pairs) {
-        ia = p.pairs[(int64_t)(fa < 0 ? 0 : fa) * p.pair_ld + col];
-        if constexpr (BATCH == 8) ib = p.pairs[(int64_t)(fb < 0 ? 0 : fb) * p.pair_ld + col];
-      } else {
-        ia = col;
-        ib = col;
-      }
-      ia = fa < 0 ? -1 : ia;
-      ib = fb < 0 ? -1 : ib;
-      // gathered rows of the batch: slot i's index for row lr sits in lane
-      // (i % 4) * 16 + lr
-      u32x4 af[BATCH][KC];
+    }
+  };
+  auto mma_batch = [&](uint32_t bm, const u32x4 (&af)[BATCH][KC], f32x4 (&acc)[NBC]) {
 #pragma unroll
-      for (int i = 0; i < BATCH; ++i) {
-        const int idx = __shfl(i < 4 ? ia : ib, (i & 3) * 16 + lr);
+    for (int i = 0; i < BATCH; ++i) {
+      if (!bm) break;
+      const int f = __builtin_ctz(bm);
+      bm &= bm - 1u;
 #pragma unroll
-        for (int kc = 0; kc < KC; ++kc) {
-          const int k = kc * CH + lg * E;
-          const uint32_t off = idx >= 0 ? (uint32_t)(((int64_t)idx * p.lda + k) * (int64_t)sizeof(T)) : kBufOOB;
-          af[i][kc] = bld16(ra, off);
-        }
-      }
+      for (int kc = 0; kc < KC; ++kc) {
 #pragma unroll
-      for (int i = 0; i < BATCH; ++i) {
-        const int f = fo[i];
-        if (f < 0) break;
-#pragma unroll
-        for (int kc = 0; kc < KC; ++kc) {
-#pragma unroll
-          for (int nb = 0; nb < NBC; ++nb) {
-            const int n = nb * 16 + lr;
-            const int c = kc * (CH / E) + lg;
-            const u32x4 bfr =
-                *reinterpret_cast<const u32x4*>(wlds + ((f * NCOL + n) * CHUNKS + (c ^ (n & SWZ))) * 16);
-            MM::step(acc[nb], af[i][kc], bfr);
-          }
+        for (int nb = 0; nb < NBC; ++nb) {
+          const int n = nb * 16 + lr;
+          const int c = kc * (CH / E) + lg;
+          const u32x4 bfr =
+              *reinterpret_cast<const u32x4*>(wlds + ((f * NCOL + n) * CHUNKS + (c ^ (n & SWZ))) * 16);
+          MM::step(acc[nb], af[i][kc], bfr);
         }
       }
     }
-
-    // epilogue: accumulator element i of column block nb is row 4*lg + i,
-    // column nb*16 + lr
+  };
+  // epilogue: accumulator element i of column block nb is row 4*lg + i,
+  // column nb*16 + lr
+  auto epilogue = [&](int t, int orow, const f32x4 (&acc)[NBC]) {
+    const int row0 = t * kMaskTile;
+    if (vec_store) {
+      OutT* st = reinterpret_cast<OutT*>(stage);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int nb = 0; nb < NBC; ++nb) {
+          const int n = n0 + nb * 16 + lr;
+          float v = acc[nb][i];
+          if (p.scale) v *= p.scale[n];
+          v = apply_act(v * p.alpha, p.act_type, p.act_alpha);
+          st[(4 * lg + i) * NCOL + nb * 16 + lr] = from_f32<OutT>(v);
+        }
+      __builtin_amdgcn_wave_barrier();
+      constexpr int LPR = ROWB / 16;   // lanes per row
+      constexpr int RPI = 64 / LPR;    // rows per store instruction (may exceed 16)
+#pragma unroll
+      for (int rb = 0; rb < 16; rb += RPI) {
+        const int rr = rb + lane / LPR;
+        const int chunk = lane % LPR;
+        const bool ok = rr < 16 && row0 + rr < p.m;
+        const int o = __shfl(orow, rr & 15);
+        if (ok) {
+          const u32x4 v = *reinterpret_cast<const u32x4*>(stage + rr * ROWB + chunk * 16);
+          *reinterpret_cast<u32x4*>(reinterpret_cast<char*>(Cout + (int64_t)o * p.ldc + n0) + chunk * 16) = v;
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      return;
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int rr = 4 * lg + i;
@@ -167,6 +238,56 @@ __global__ __launch_bounds__(1024) void rowgather_ws_kernel(RowGatherArgs p, int
         }
       }
     }
+  };
+
+  // Tiles t, t + G, t + 2G, ... of this wave, software-pipelined across
+  // tiles: while the gathered rows of tile k are in flight, the pair indices
+  // of tile k+1 and the row data of tile k+2 load too, so a tile costs about
+  // one round trip.  Tiles with more than BATCH offsets finish their extra
+  // batches inline.
+  int t = blockIdx.x * kWsWaves + wave;
+  if (t >= ntiles) return;
+  Meta Mc, Mn;
+  load_meta(t, Mc);
+  bool live_c;
+  uint32_t rem_c = prepare(tile_mask(t, Mc.m), live_c);
+  int ia_c, ib_c;
+  uint32_t bm_c = batch(rem_c, Mc.col, ia_c, ib_c);
+  load_meta(t + G, Mn);
+  while (true) {
+    u32x4 af[BATCH][KC];
+    gather(bm_c, ia_c, ib_c, af);
+    // next tile: its mask is final once its row data has landed
+    const int tn = t + G;
+    bool live_n = false;
+    uint32_t rem_n = 0u, bm_n = 0u;
+    int ia_n = -1, ib_n = -1;
+    Meta Mnn;
+    if (tn < ntiles) {
+      rem_n = prepare(tile_mask(tn, Mn.m), live_n);
+      bm_n = batch(rem_n, Mn.col, ia_n, ib_n);
+      load_meta(tn + G, Mnn);
+    }
+    f32x4 acc[NBC];
+#pragma unroll
+    for (int nb = 0; nb < NBC; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    mma_batch(bm_c, af, acc);
+    while (rem_c) {
+      int ia, ib;
+      const uint32_t bm = batch(rem_c, Mc.col, ia, ib);
+      gather(bm, ia, ib, af);
+      mma_batch(bm, af, acc);
+    }
+    if (live_c) epilogue(t, Mc.orow, acc);
+    if (tn >= ntiles) break;
+    t = tn;
+    Mc = Mn;
+    Mn = Mnn;
+    live_c = live_n;
+    rem_c = rem_n;
+    bm_c = bm_n;
+    ia_c = ia_n;
+    ib_c = ib_n;
   }
 }
 
