@@ -10,10 +10,17 @@ Reported ``value``: effective TFLOP/s of the FP16 step (FP16 in, FP32
 accumulate), FLOPs = 3 * 2 * P * C * K with P = count(pair_fwd != -1).
 ``per_dtype`` gives FP32 / FP16 / BF16 TFLOP/s, fraction of the MI355X dense
 MFMA peak and the speedup over the plain FP32 path on the same GPU.
-``roofline`` covers the step's dominant kernel, timed with HIP events on its
-stream.  ``cpu_baseline`` times the reference's CPU path (numpy FP32
-gather-GEMM-scatter over compact pairs) on a bounded sample on the host, and
-records the FP64-oracle error of every GPU result.
+``roofline`` covers the step's dominant kernel, the row-gather implicit GEMM
+(launched once by forward and once by dgrad), timed with HIP events on its
+stream around the forward launch; ``traffic`` is its measured HBM bytes per
+launch, read from the counter summary in profiles/ when that summary was taken
+on the same workload.  ``cpu_baseline`` times the reference's CPU path (numpy
+FP32 gather-GEMM-scatter over compact pairs) on a bounded sample on the host,
+and ``accuracy_vs_fp64`` records the FP64-oracle error of every GPU result.
+
+Profiling helpers: ``--ops fwd`` restricts the step to the listed ops and
+``--copy-bytes N`` first runs a device-to-device copy of N bytes three times,
+the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters.
 """
 from __future__ import annotations
 
@@ -39,17 +46,33 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--voxels", type=int, default=120000)
-    ap.add_argument("--channels", type=int, default=64)
+    ap.add_argument("--config", choices=("c2", "c3"), default="c2",
+                    help="c2: 120k voxels 64->64 (default); c3: precision form, 200k voxels 128->128")
+    ap.add_argument("--voxels", type=int, default=None)
+    ap.add_argument("--channels", type=int, default=None)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--dtypes", default="f32,f16,bf16")
-    return ap.parse_args()
+    ap.add_argument("--dtypes", default=None)
+    ap.add_argument("--ops", default="fwd,dgrad,wgrad")
+    ap.add_argument("--copy-bytes", type=int, default=0)
+    args = ap.parse_args()
+    dflt = {"c2": (120000, 64, "f32,f16,bf16"), "c3": (200000, 128, "f32,bf16,f16,e4m3")}[args.config]
+    args.voxels = args.voxels or dflt[0]
+    args.channels = args.channels or dflt[1]
+    args.dtypes = args.dtypes or dflt[2]
+    return args
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def out_bytes(dt: str, op: str) -> int:
+    """Output element size: FP8 forward / dgrad write f16, FP8 wgrad FP32."""
+    if dt == "e4m3":
+        return 4 if op == "wgrad" else 2
+    return DT_BYTES[dt]
 
 
 def op_bytes(op: str, n: int, kv: int, C: int, K: int, s: int, s_out: int) -> int:
@@ -65,29 +88,52 @@ def op_bytes(op: str, n: int, kv: int, C: int, K: int, s: int, s_out: int) -> in
 class Layer:
     """Device-resident buffers and prebuilt ConvParams for one dtype."""
 
-    def __init__(self, prob, X, W, dO, dt: str, stream: int):
+    def __init__(self, prob, X, W, dO, dt: str, stream: int, ops=("fwd", "dgrad", "wgrad")):
         import torch
         from cumm_amd import tensorview as tv
         from cumm_amd.conv.main import ConvMainUnitTest, select_conv_algo
         from cumm_amd.tensorview.gemm import ConvOpType, ConvParams
 
-        tdt = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16}[dt]
-        tvdt = {"f32": tv.float32, "f16": tv.float16, "bf16": tv.bfloat16}[dt]
+        TD = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16, "e4m3": torch.float8_e4m3fn}
+        TV = {"f32": tv.float32, "f16": tv.float16, "bf16": tv.bfloat16, "e4m3": tv.float_e4m3}
         n, C, K, kv = prob.n, X.shape[1], W.shape[0], prob.kv
         dev = "cuda"
-        self.x = torch.from_numpy(X).to(dev, tdt)
-        self.w = torch.from_numpy(W).to(dev, tdt)
-        self.dout = torch.from_numpy(dO).to(dev, tdt)
-        self.out = torch.empty((n, K), dtype=tdt, device=dev)
-        self.din = torch.zeros((n, C), dtype=tdt, device=dev)
-        self.dw = torch.empty((K, kv, C), dtype=tdt, device=dev)
+        self.scale_f = self.scale_d = None
+        self.alpha_w = 1.0
+        if dt == "e4m3":
+            # FP8-E4M3 operands with absmax scales (448 = largest finite e4m3):
+            # forward weights per output channel k, dgrad weights per output
+            # channel c, features and gradients per tensor; FP32 accumulation,
+            # f16 outputs for forward / dgrad, FP32 dW.
+            s_x = float(np.abs(X).max()) / 448.0
+            s_do = float(np.abs(dO).max()) / 448.0
+            s_wk = np.abs(W).reshape(K, -1).max(axis=1) / 448.0
+            s_wc = np.abs(W).max(axis=(0, 1)) / 448.0
+            q = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(dev).to(TD["e4m3"])  # noqa: E731
+            self.x = q(X / s_x)
+            self.w = q(W / s_wk[:, None, None])
+            self.w_d = q(W / s_wc[None, None, :])
+            self.dout = q(dO / s_do)
+            self.scale_f = torch.from_numpy((s_x * s_wk).astype(np.float32)).to(dev)
+            self.scale_d = torch.from_numpy((s_do * s_wc).astype(np.float32)).to(dev)
+            self.alpha_w = s_do * s_x
+            o_fd, o_w = "f16", "f32"
+        else:
+            self.x = torch.from_numpy(X).to(dev, TD[dt])
+            self.w = torch.from_numpy(W).to(dev, TD[dt])
+            self.w_d = self.w
+            self.dout = torch.from_numpy(dO).to(dev, TD[dt])
+            o_fd = o_w = dt
+        self.out = torch.empty((n, K), dtype=TD[o_fd], device=dev)
+        self.din = torch.zeros((n, C), dtype=TD[o_fd], device=dev)
+        self.dw = torch.empty((K, kv, C), dtype=TD[o_w], device=dev)
         pf = torch.from_numpy(prob.pair_fwd).to(dev)
         pb = torch.from_numpy(prob.pair_bwd).to(dev)
         ms = torch.from_numpy(prob.mask_sorted.view(np.int32)).to(dev)
         am = torch.from_numpy(prob.argsort).to(dev)
-        fd = select_conv_algo(0, tvdt, tvdt, tvdt, C, K)
-        dd = select_conv_algo(1, tvdt, tvdt, tvdt, C, K)
-        wd = select_conv_algo(2, tvdt, tvdt, tvdt, C, K)
+        fd = select_conv_algo(0, TV[dt], TV[dt], TV[o_fd], C, K)
+        dd = select_conv_algo(1, TV[o_fd], TV[dt], TV[dt], C, K)
+        wd = select_conv_algo(2, TV[dt], TV[o_w], TV[dt], C, K)
         tile = fd.tile_shape[0]
         self.mask_out = torch.zeros(((n + tile - 1) // tile,), dtype=torch.int32, device=dev)
 
@@ -101,12 +147,17 @@ class Layer:
             if op == 0:
                 p.input, p.weight, p.output = tv.from_torch(self.x), tv.from_torch(self.w), tv.from_torch(self.out)
                 p.mask_output = tv.from_torch(self.mask_out)
+                if self.scale_f is not None:
+                    p.scale = tv.from_torch(self.scale_f)
             elif op == 1:
-                p.input, p.weight, p.output = tv.from_torch(self.din), tv.from_torch(self.w), tv.from_torch(self.dout)
+                p.input, p.weight, p.output = tv.from_torch(self.din), tv.from_torch(self.w_d), tv.from_torch(self.dout)
                 p.reverse_mask = True
+                if self.scale_d is not None:
+                    p.scale = tv.from_torch(self.scale_d)
             else:
                 p.input, p.weight, p.output = tv.from_torch(self.x), tv.from_torch(self.dw), tv.from_torch(self.dout)
                 p.mask_width = tile
+                p.alpha = self.alpha_w
             ws = ConvMainUnitTest.query_workspace_size(p)
             if ws > 0:
                 p.workspace = tv.from_torch(torch.empty((ws,), dtype=torch.uint8, device=dev))
@@ -115,11 +166,11 @@ class Layer:
         self.params = {"fwd": params(0, fd), "dgrad": params(1, dd), "wgrad": params(2, wd)}
         self.names = {k: repr(v.conv_algo_desp) for k, v in self.params.items()}
         self.run = ConvMainUnitTest.implicit_gemm2
+        self.ops = [k for k in ("fwd", "dgrad", "wgrad") if k in ops]
 
     def step(self):
-        self.run(self.params["fwd"])
-        self.run(self.params["dgrad"])
-        self.run(self.params["wgrad"])
+        for k in self.ops:
+            self.run(self.params[k])
 
 
 def time_dtype(layer: Layer, steps: int, warmup: int, stream):
@@ -128,20 +179,21 @@ def time_dtype(layer: Layer, steps: int, warmup: int, stream):
     for _ in range(warmup):
         layer.step()
     torch.cuda.synchronize()
-    # per-op device time with HIP events on the launch stream
-    ev = {k: [] for k in ("fwd", "dgrad", "wgrad")}
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        for k in ("fwd", "dgrad", "wgrad"):
-            a = torch.cuda.Event(enable_timing=True)
-            b = torch.cuda.Event(enable_timing=True)
-            a.record(stream)
+    # Per-op device time with HIP events on the launch stream: `steps`
+    # back-to-back launches of one op between two events, so the queue stays
+    # ahead of the GPU and the average is the kernels' own duration.
+    mean_op = {}
+    for k in layer.ops:
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record(stream)
+        for _ in range(steps):
             layer.run(layer.params[k])
-            b.record(stream)
-            ev[k].append((a, b))
-    torch.cuda.synchronize()
-    wall_ev = time.perf_counter() - t0
-    per_op = {k: float(np.median([a.elapsed_time(b) for a, b in v])) for k, v in ev.items()}
+        b.record(stream)
+        torch.cuda.synchronize()
+        mean_op[k] = a.elapsed_time(b) / steps
+    per_op = dict(mean_op)
     # whole-step wall clock, synchronize on both sides, no events inside
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -149,7 +201,36 @@ def time_dtype(layer: Layer, steps: int, warmup: int, stream):
         layer.step()
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) / steps
-    return wall, per_op
+    return wall, per_op, mean_op
+
+
+def copy_calibration(nbytes: int):
+    """A device-to-device copy of nbytes, three times: the known-size kernel
+    whose FETCH_SIZE / WRITE_SIZE readings scale the counters of the others."""
+    import torch
+
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copy calibration: {nbytes} bytes x 3")
+
+
+def load_traffic(kernel: str, workload: str):
+    """Measured HBM bytes per launch of `kernel` on `workload` from the counter
+    summary (profiles/traffic.json), or None when none was taken for them."""
+    f = ROOT / "profiles" / "traffic.json"
+    if not f.exists():
+        return None
+    try:
+        rows = json.loads(f.read_text())
+    except ValueError:
+        return None
+    for r in rows:
+        if r.get("kernel") == kernel and r.get("workload") == workload:
+            return r
+    return None
 
 
 def cpu_baseline(prob, X, W, dO, budget_s: float, gpu_outputs):
@@ -192,7 +273,7 @@ def cpu_baseline(prob, X, W, dO, budget_s: float, gpu_outputs):
         acc[dt_name] = e
     return {
         "value": flops_step / dt / 1e12, "unit": "TFLOP/s", "cores": int(threads), "kind": "port",
-        "sample": f"{steps} full fwd+dgrad+wgrad steps of the same 120k-voxel 64->64 layer "
+        "sample": f"{steps} full fwd+dgrad+wgrad steps of the same {n}-voxel {C}->{K} layer "
                   f"(numpy FP32, compact pairs), {dt * 1e3:.0f} ms/step",
         "ms_per_step": dt * 1e3,
     }, acc
@@ -221,14 +302,17 @@ def main():
     flops_op = 2 * P * C * K
     flops_step = 3 * flops_op
 
+    if args.copy_bytes > 0:
+        copy_calibration(args.copy_bytes)
     stream = torch.cuda.current_stream()
     sid = stream.cuda_stream
+    ops = tuple(args.ops.split(","))
     results = {}
     gpu_outputs = {}
     for dt in args.dtypes.split(","):
-        layer = Layer(prob, X, W, dO, dt, sid)
-        wall, per_op = time_dtype(layer, args.steps, args.warmup, stream)
-        results[dt] = (wall, per_op, layer.names)
+        layer = Layer(prob, X, W, dO, dt, sid, ops)
+        wall, per_op, mean_op = time_dtype(layer, args.steps, args.warmup, stream)
+        results[dt] = (wall, per_op, layer.names, mean_op)
         torch.cuda.synchronize()
         gpu_outputs[dt] = (layer.out.double().cpu().numpy(), layer.din.double().cpu().numpy(),
                            layer.dw.double().cpu().numpy(), layer.mask_out.cpu().numpy().view(np.uint32))
@@ -236,10 +320,11 @@ def main():
         del layer
 
     main_dt = "f16" if "f16" in results else list(results)[0]
-    wall, per_op, names = results[main_dt]
+    wall, per_op, names, mean_op = results[main_dt]
+    flops_step = len(ops) * flops_op
     f32_tf = flops_step / results["f32"][0] / 1e12 if "f32" in results else None
     per_dtype = {}
-    for dt, (w, po, nm) in results.items():
+    for dt, (w, po, nm, _) in results.items():
         tf = flops_step / w / 1e12
         per_dtype[dt] = {
             "tflops": tf, "ms_per_step": w * 1e3, "frac_mfma_peak": tf * 1e12 / PEAK[dt],
@@ -247,16 +332,31 @@ def main():
             "op_us": {k: v * 1e3 for k, v in po.items()},
             "op_tflops": {k: flops_op / (v * 1e-3) / 1e12 for k, v in po.items()},
         }
-    # dominant kernel of the reported step
-    dom = max(per_op, key=per_op.get)
+    workload = (f"subm3x3x3 fwd+dgrad+wgrad, {prob.n} voxels on 8 planes in 41x1600x1408, "
+                f"{C}->{K} ch")
+    # Dominant kernel of the step: the row-gather implicit GEMM, launched by
+    # forward and by dgrad (one kernel each); the forward launch is timed alone.
     s = DT_BYTES[main_dt]
-    byt = op_bytes(dom, prob.n, prob.kv, C, K, s, s)
-    t_dom = per_op[dom] * 1e-3
-    ach_bw = byt / t_dom
-    roofline = {"kernel": names[dom], "op": dom, "bound": "hbm", "achieved": ach_bw / 1e9,
-                "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach_bw / HBM_PEAK, "traffic": None,
-                "algorithmic_bytes": byt, "avg_us": per_op[dom] * 1e3,
-                "mfma_tflops": flops_op / t_dom / 1e12}
+    rop = "fwd" if "fwd" in mean_op else ops[0]
+    byt = op_bytes(rop, prob.n, prob.kv, C, K, s, out_bytes(main_dt, rop))
+    t_k = mean_op[rop] * 1e-3
+    ach_bw = byt / t_k
+    kname = {"fwd": "rowgather_ws_kernel", "dgrad": "rowgather_ws_kernel", "wgrad": "wgrad_kernel"}[rop]
+    tr = load_traffic(f"{kname}<{main_dt}> {rop}", workload)
+    roofline = {"kernel": f"{kname}<{main_dt}> ({rop})", "desp": names[rop], "bound": "hbm",
+                "achieved": ach_bw / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach_bw / HBM_PEAK,
+                "traffic": tr["traffic_bytes"] if tr else None,
+                "algorithmic_bytes": byt, "avg_us": mean_op[rop] * 1e3,
+                "mfma_tflops": flops_op / t_k / 1e12, "mfma_frac": flops_op / t_k / PEAK[main_dt]}
+    # every op of the reported dtype against its own roof (ops of several kernels
+    # are timed as a whole: dgrad = weight transpose + row gather, wgrad = plan +
+    # partial products + fixed-order reduce)
+    roofline_ops = {}
+    for k, v in mean_op.items():
+        b = op_bytes(k, prob.n, prob.kv, C, K, s, out_bytes(main_dt, k))
+        roofline_ops[k] = {"avg_us": v * 1e3, "algorithmic_bytes": b, "GB/s": b / (v * 1e-3) / 1e9,
+                           "frac_hbm": b / (v * 1e-3) / HBM_PEAK,
+                           "frac_mfma": flops_op / (v * 1e-3) / PEAK[main_dt]}
 
     cpu = None
     accuracy = None
@@ -271,11 +371,11 @@ def main():
         "ms_per_step": wall * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": main_dt, "data": "synthetic",
-        "config": {"workload": f"subm3x3x3 fwd+dgrad+wgrad, {prob.n} voxels on 8 planes in 41x1600x1408, "
-                               f"{C}->{K} ch", "voxels": prob.n, "pairs": P, "channels_in": C,
+        "config": {"workload": workload, "ops": list(ops), "voxels": prob.n, "pairs": P, "channels_in": C,
                    "channels_out": K, "kernel_volume": prob.kv, "flops_per_step": flops_step},
         "per_dtype": per_dtype,
         "roofline": roofline,
+        "roofline_ops": roofline_ops,
         "cpu_baseline": cpu,
         "accuracy_vs_fp64": accuracy,
     }

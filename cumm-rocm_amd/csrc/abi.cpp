@@ -360,7 +360,7 @@ const std::vector<cs_conv_algo_desp>& conv_catalogue() {
         v.push_back(conv_desp(1, rowgather_desp(c.in, c.w, c.out, cs::kMaskTile, bn, 0, 0, 0)));
       }
     }
-    const ConvCombo wg_combos[] = {{0, 0, 0}, {7, 7, 7}, {12, 12, 12}, {7, 7, 0}, {12, 12, 0}};
+    const ConvCombo wg_combos[] = {{0, 0, 0}, {7, 7, 7}, {12, 12, 12}, {7, 7, 0}, {12, 12, 0}, {14, 14, 0}};
     for (auto c : wg_combos) {
       // wgrad: a = dout, b = input, c = dW; trans (T, F, F)
       v.push_back(conv_desp(2, wgrad_desp(c.in, c.w, c.out, cs::kMaskTile)));

@@ -41,3 +41,52 @@ def conv_forward_coords(in_coords, X, W, out_coords=None, stride: int = 1, paddi
                     s += W[k, f, c] * X[i, c]
                 out[o, k] += s
     return out
+
+
+def _neighbours(in_coords, out_coords, stride: int, padding: int, subm: bool):
+    """Yield (o, f, i) for every output row o, offset f and input row i that the
+    coordinate rule connects (same enumeration as conv_forward_coords)."""
+    lut = {tuple(int(v) for v in c): i for i, c in enumerate(in_coords)}
+    outs = in_coords if subm else out_coords
+    offs = list(itertools.product(range(3), range(3), range(3)))
+    for o, oc in enumerate(outs):
+        b = int(oc[0])
+        for f, (kz, ky, kx) in enumerate(offs):
+            if subm:
+                nb = (b, int(oc[1]) + kz - 1, int(oc[2]) + ky - 1, int(oc[3]) + kx - 1)
+            else:
+                nb = (b, int(oc[1]) * stride - padding + kz, int(oc[2]) * stride - padding + ky,
+                      int(oc[3]) * stride - padding + kx)
+            i = lut.get(nb)
+            if i is not None:
+                yield o, f, i
+
+
+def conv_dgrad_coords(in_coords, dO, W, out_coords=None, stride: int = 1, padding: int = 1,
+                      subm: bool = True) -> np.ndarray:
+    """dX[i] = sum over (o, f) connected to i of W[:, f, :]^T dO[o], by loops."""
+    dO = np.asarray(dO, dtype=np.float64)
+    W = np.asarray(W, dtype=np.float64)
+    K, kv, C = W.shape
+    dx = np.zeros((len(in_coords), C))
+    for o, f, i in _neighbours(in_coords, out_coords, stride, padding, subm):
+        for c in range(C):
+            s = 0.0
+            for k in range(K):
+                s += W[k, f, c] * dO[o, k]
+            dx[i, c] += s
+    return dx
+
+
+def conv_wgrad_coords(in_coords, X, dO, kv: int, out_coords=None, stride: int = 1, padding: int = 1,
+                      subm: bool = True) -> np.ndarray:
+    """dW[k, f, c] = sum over connected (o, i) of dO[o, k] * X[i, c], by loops."""
+    X = np.asarray(X, dtype=np.float64)
+    dO = np.asarray(dO, dtype=np.float64)
+    K, C = dO.shape[1], X.shape[1]
+    dw = np.zeros((K, kv, C))
+    for o, f, i in _neighbours(in_coords, out_coords, stride, padding, subm):
+        for k in range(K):
+            for c in range(C):
+                dw[k, f, c] += dO[o, k] * X[i, c]
+    return dw

@@ -275,3 +275,53 @@ def test_gemm_emulator_config1():
     c2 = np.zeros((128, 128), np.float32)
     L.gemm_emu_f32(a2.ctypes.data, b2.ctypes.data, c2.ctypes.data, 128, 128, k)
     assert np.max(np.abs(c2 - a2.astype(np.float64) @ b2.T.astype(np.float64))) < 1e-4
+
+
+# --------------------------------------------------------------- golden fixtures
+GOLD = ROOT / "tests" / "golden"
+
+
+def test_golden_subm_fixture():
+    """Pair builder bit-exact against the stored tables; the FP64 mask-walk
+    oracle equal to the coordinate brute force stored in the fixture."""
+    g = np.load(GOLD / "subm_tiny.npz")
+    prob = pc.build_subm(g["coords"], (6, 14, 14))
+    for name in ("pair_fwd", "pair_bwd", "mask", "mask_sorted", "argsort"):
+        assert np.array_equal(getattr(prob, name), g[name]), name
+    assert np.array_equal(pc.tile_masks(prob.mask_sorted, 16), g["tile_mask16"])
+    X, W, dO = (g[k].astype(np.float64) for k in ("X", "W", "dO"))
+    out, tm = ref.conv_forward(X, W, g["pair_fwd"], g["mask_sorted"], g["argsort"], 16)
+    assert np.array_equal(tm, g["tile_mask16"])
+    assert np.allclose(out, g["out"], rtol=0, atol=1e-12)
+    din = ref.conv_dgrad(dO, W, g["pair_bwd"], g["mask_sorted"], g["argsort"], 16, reverse_mask=True)
+    assert np.allclose(din, g["din"], rtol=0, atol=1e-12)
+    dw = ref.conv_wgrad(X, dO, g["pair_fwd"], g["tile_mask16"], g["argsort"], 16)
+    assert np.allclose(dw, g["dw"], rtol=0, atol=1e-11)
+
+
+def test_golden_strided_fixture():
+    g = np.load(GOLD / "strided_tiny.npz")
+    sp = pc.build_strided(g["in_coords"], (6, 14, 14))
+    for name in ("out_coords", "pair_fwd", "pair_bwd", "mask_fwd_sorted", "argsort_fwd", "mask_bwd_sorted",
+                 "argsort_bwd"):
+        assert np.array_equal(getattr(sp, name), g[name]), name
+    X, W, dO = (g[k].astype(np.float64) for k in ("X", "W", "dO"))
+    out, tm = ref.conv_forward(X, W, g["pair_fwd"], g["mask_fwd_sorted"], g["argsort_fwd"], 16)
+    assert np.allclose(out, g["out"], rtol=0, atol=1e-12)
+    din = ref.conv_dgrad(dO, W, g["pair_bwd"], g["mask_bwd_sorted"], g["argsort_bwd"], 16)
+    assert np.allclose(din, g["din"], rtol=0, atol=1e-12)
+    dw = ref.conv_wgrad(X, dO, g["pair_fwd"], tm, g["argsort_fwd"], 16)
+    assert np.allclose(dw, g["dw"], rtol=0, atol=1e-11)
+
+
+def test_golden_gemm_fixture():
+    g = np.load(GOLD / "gemm_small.npz")
+    got = gemm_ref.matmul2(g["a"], g["b"], np.zeros((128, 128)))
+    assert np.allclose(got, g["c"], rtol=0, atol=1e-12)
+    m, n = g["ac_c"].shape
+    got = gemm_ref.matmul2(g["ac_a"], g["ac_b"], np.zeros((m, n)), shuffle=1, a_inds=g["ac_a_inds"],
+                           c_inds=g["ac_c_inds"])
+    assert np.allclose(got, g["ac_c"], rtol=0, atol=1e-12)
+    got = gemm_ref.matmul2(g["ab_a"], g["ab_b"], np.zeros(g["ab_c"].shape), trans_a=True, shuffle=2,
+                           a_inds=g["ab_a_inds"], b_inds=g["ab_b_inds"])
+    assert np.allclose(got, g["ab_c"], rtol=0, atol=1e-12)

@@ -198,3 +198,50 @@ def test_forward_fp8_per_channel_scales(case64, out_dt):
     mx, rel = ref.relative_error(got, full)
     print(f"[fp8] out={out_dt} error vs FP64 on unquantised inputs: rel={rel:.3e} max_abs={mx:.3e}")
     assert rel < 0.1
+
+
+def test_dgrad_fp8_per_channel_scales(case64):
+    """FP8 dgrad: dout per-tensor scale, weights quantised per input channel c
+    (the dgrad output channel), scale[c] = s_do * s_w[c] in the epilogue."""
+    c = case64
+    prob = c.prob
+    s_do = np.abs(c.dO).max() / 448.0
+    s_w = np.abs(c.W).max(axis=(0, 1)) / 448.0
+    dq, dq64 = _e4m3_quant(c.dO, s_do)
+    wq, wq64 = _e4m3_quant(c.W, s_w[None, None, :])
+    scale = (s_do * s_w).astype(np.float32)
+    desp = desp_for(1, "e4m3", c.C, c.K, "f16")
+    din = torch.zeros((prob.n, c.C), dtype=torch.float16, device="cuda")
+    p = conv_params(1, desp, din, wq.cuda(), dq.cuda(), prob.pair_bwd, prob.mask_sorted, prob.argsort,
+                    reverse_mask=True, scale=torch.from_numpy(scale).cuda())
+    ConvMainUnitTest.implicit_gemm2(p)
+    torch.cuda.synchronize()
+    exp = ref.conv_dgrad(dq64, wq64, prob.pair_bwd, prob.mask_sorted, prob.argsort, 16, reverse_mask=True)
+    exp = exp * scale[None, :]
+    assert _rel(din.double().cpu().numpy(), quantize(exp, "f16")) <= TOL["f16"]
+
+
+def test_wgrad_fp8_deterministic():
+    """FP8 wgrad: dout and features per-tensor scaled, alpha = s_do * s_x, FP32
+    dW from the fixed-order tree; two runs bit-identical.  Bar 1e-4 relative
+    against FP64 on the dequantised operands: the FP8 MFMA sums its 32
+    products before the FP32 accumulate (measured 1.3e-5)."""
+    c = subm_case(3000, 64, 64, seed=3)
+    prob = c.prob
+    _, mo, fdesp = run_fwd(c, "f32")
+    s_x = np.abs(c.X).max() / 448.0
+    s_do = np.abs(c.dO).max() / 448.0
+    xq, xq64 = _e4m3_quant(c.X, s_x)
+    dq, dq64 = _e4m3_quant(c.dO, s_do)
+    desp = desp_for(2, "e4m3", c.C, c.K, "f32")
+    res = []
+    for _ in range(2):
+        dw = torch.zeros((c.K, prob.kv, c.C), dtype=torch.float32, device="cuda")
+        p = conv_params(2, desp, xq.cuda(), dw, dq.cuda(), prob.pair_fwd, mo, prob.argsort, mask_width=16,
+                        alpha=float(s_do * s_x))
+        ConvMainUnitTest.implicit_gemm2(p)
+        torch.cuda.synchronize()
+        res.append(dw.cpu().numpy())
+    assert np.array_equal(res[0].view(np.int32), res[1].view(np.int32))
+    exp = ref.conv_wgrad(xq64, dq64, prob.pair_fwd, mo, prob.argsort, 16) * (s_do * s_x)
+    assert _rel(res[0].astype(np.float64), exp) <= 1e-4
