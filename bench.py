@@ -46,8 +46,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", choices=("c2", "c3"), default="c2",
-                    help="c2: 120k voxels 64->64 (default); c3: precision form, 200k voxels 128->128")
+    ap.add_argument("--config", choices=("c2", "c3", "c4"), default="c2",
+                    help="c2: 120k voxels 64->64 (default); c3: precision form, 200k voxels 128->128; "
+                         "c4: 32 clouds, strided 32->64 + subm 64->128, grouped launches under a HIP graph")
     ap.add_argument("--voxels", type=int, default=None)
     ap.add_argument("--channels", type=int, default=None)
     ap.add_argument("--seed", type=int, default=0)
@@ -57,7 +58,8 @@ def parse():
     ap.add_argument("--ops", default="fwd,dgrad,wgrad")
     ap.add_argument("--copy-bytes", type=int, default=0)
     args = ap.parse_args()
-    dflt = {"c2": (120000, 64, "f32,f16,bf16"), "c3": (200000, 128, "f32,bf16,f16,e4m3")}[args.config]
+    dflt = {"c2": (120000, 64, "f32,f16,bf16"), "c3": (200000, 128, "f32,bf16,f16,e4m3"),
+            "c4": (0, 0, "f16")}[args.config]
     args.voxels = args.voxels or dflt[0]
     args.channels = args.channels or dflt[1]
     args.dtypes = args.dtypes or dflt[2]
@@ -279,11 +281,140 @@ def cpu_baseline(prob, X, W, dO, budget_s: float, gpu_outputs):
     }, acc
 
 
+def _clouds_worker(job):
+    """Host-side scene and pair tables of one cloud (runs in a worker process)."""
+    from cumm_amd import batch as B
+
+    from cumm_amd import pointcloud as pc
+
+    size, seed, b = job
+    c = pc.plane_scene(size, seed=seed, batch_index=b)
+    return c, B.build_stack_tables(("strided", "subm"), c, pc.GRID_KITTI)
+
+
+def run_batched(args):
+    """Batched form: 32 clouds (log-uniform 5k..150k voxels) through a strided
+    3x3x3 s2 32->64 layer (outputs = unique(floor(c / 2))) and a subm 64->128
+    layer, forward + backward (dgrad of the second layer, wgrad of both).
+    Compared: the plain FP32 per-call path (one launch per cloud, layer and op),
+    FP16 per cloud, FP16 grouped (one launch per layer and op over all clouds)
+    and FP16 grouped with tuned kernels replayed from one HIP graph."""
+    from concurrent.futures import ProcessPoolExecutor
+
+    from cumm_amd import batch as B
+    from cumm_amd import pointcloud as pc
+
+    t0 = time.perf_counter()
+    sizes = B.cloud_sizes(32, 5000, 150000, args.seed)
+    jobs = [(s, args.seed + 7919 * b, b) for b, s in enumerate(sizes)]
+    with ProcessPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        res = list(ex.map(_clouds_worker, jobs))
+    clouds = [r[0] for r in res]
+    per_cloud = [r[1] for r in res]
+    coords = np.concatenate(clouds)
+    tabs = B.build_stack_tables(("strided", "subm"), coords, pc.GRID_KITTI)
+    log(f"c4: {len(sizes)} clouds, {coords.shape[0]} voxels; layer outputs "
+        f"{[t.n_out for t in tabs]}, pairs {[t.num_pairs for t in tabs]}; host tables "
+        f"{time.perf_counter() - t0:.1f}s")
+    import torch
+    from cumm_amd.tune import ConvTuner
+
+    chans = (32, 64, 128)
+    rng = np.random.default_rng(args.seed + 1)
+    Ws = [rng.normal(0.0, 1.0 / np.sqrt(27 * chans[i]), size=(chans[i + 1], 27, chans[i])).astype(np.float32)
+          for i in range(2)]
+    X = rng.standard_normal((coords.shape[0], chans[0])).astype(np.float32)
+    dO = rng.standard_normal((tabs[-1].n_out, chans[-1])).astype(np.float32)
+    in_off = np.cumsum([0] + [c.shape[0] for c in clouds])
+    out_off = np.cumsum([0] + [t[-1].n_out for t in per_cloud])
+    stream = torch.cuda.current_stream()
+    sid = stream.cuda_stream
+
+    def stack(tables, dt, x, d):
+        st = B.ConvStack(tables, chans, dt, Ws, sid, train=True)
+        st.bind(torch.from_numpy(x).cuda().to(B.TORCH_DT[dt]), torch.from_numpy(d).cuda().to(B.TORCH_DT[dt]))
+        return st
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / steps
+
+    flops = None
+    out = {}
+    for dt in ("f32", "f16"):
+        sts = [stack(per_cloud[b], dt, X[in_off[b]:in_off[b + 1]], dO[out_off[b]:out_off[b + 1]])
+               for b in range(len(clouds))]
+        flops = sum(st.flops() for st in sts)
+        t = timed(lambda: [st.step() for st in sts], args.steps, args.warmup)
+        out[f"{dt}_per_cloud"] = t
+        log(f"c4 {dt} per-cloud eager: {t * 1e3:.3f} ms/step")
+        if dt == "f16":
+            per_cloud_y = [st.layers[-1].out.clone() for st in sts]
+            per_cloud_dw = [st.layers[0].dw.double() for st in sts]
+        del sts
+    g = stack(tabs, "f16", X, dO)
+    assert g.flops() == flops
+    out["f16_grouped"] = timed(g.step, args.steps, args.warmup)
+    log(f"c4 f16 grouped eager: {out['f16_grouped'] * 1e3:.3f} ms/step")
+    y_eager = g.layers[-1].out.clone()
+    dw_eager = [l.dw.clone() for l in g.layers]
+    din_eager = g.layers[1].din.clone()
+    tuner = ConvTuner()
+    choices = g.tune(tuner)
+    for c in choices:
+        log(f"c4 tuned {c[0]} {c[1]}->{c[2]}: {c[3]} split {c[4]} {c[5]:.1f}us")
+    g.step()
+    torch.cuda.synchronize()
+    y_tuned = g.layers[-1].out.clone()
+    g.capture()
+    out["f16_grouped_graph"] = timed(g.replay, args.steps, args.warmup)
+    log(f"c4 f16 grouped tuned graph: {out['f16_grouped_graph'] * 1e3:.3f} ms/step")
+    torch.cuda.synchronize()
+    checks = {
+        "graph_equals_eager_tuned_fwd": bool(torch.equal(g.layers[-1].out, y_tuned)),
+        "tuned_equals_default_fwd": bool(torch.equal(y_tuned, y_eager)),
+        "grouped_equals_per_cloud_fwd": bool(all(
+            torch.equal(y_eager[out_off[b]:out_off[b + 1]], per_cloud_y[b]) for b in range(len(clouds)))),
+        "dgrad_graph_equals_eager": bool(torch.equal(g.layers[1].din, din_eager)),
+    }
+    # wgrad of the first layer: grouped vs the sum of the per-cloud results
+    dw_sum = torch.stack(per_cloud_dw).sum(0)
+    rel = float((dw_eager[0].double() - dw_sum).norm() / dw_sum.norm())
+    checks["wgrad_grouped_vs_sum_per_cloud_rel"] = rel
+    tf = {k: flops / v / 1e12 for k, v in out.items()}
+    line = {
+        "metric": "sparse-conv TFLOP/s + % MFMA peak per dtype vs plain FP32; serving p50/p99 ms",
+        "value": tf["f16_grouped_graph"], "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": out["f16_grouped_graph"] * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f16", "data": "synthetic",
+        "config": {"workload": f"32 clouds log-uniform[5k,150k] ({coords.shape[0]} voxels), strided 3x3x3 s2 "
+                               f"32->64 (unique(floor(c/2))) + subm 64->128, fwd+dgrad+wgrad",
+                   "clouds": len(sizes), "voxels": int(coords.shape[0]),
+                   "layer_out_rows": [t.n_out for t in tabs], "pairs": [t.num_pairs for t in tabs],
+                   "flops_per_step": flops},
+        "forms_ms": {k: v * 1e3 for k, v in out.items()},
+        "forms_tflops": tf,
+        "speedup_vs_f32_per_call": {k: out["f32_per_cloud"] / v for k, v in out.items()},
+        "frac_mfma_peak_f16": tf["f16_grouped_graph"] * 1e12 / PEAK["f16"],
+        "tuned": [{"op": c[0], "C": c[1], "K": c[2], "kernel": c[3], "split_k": c[4], "us": c[5]} for c in choices],
+        "checks": checks,
+    }
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.config == "c4":
+        return run_batched(args)
     import torch
     from cumm_amd import pointcloud as pc
 

@@ -3,8 +3,9 @@
 //
 // A block owns 64 sorted rows (four 16-row mask tiles) and 64 output columns.
 // Its four waves OR the four tile masks into the same scalar union and split
-// the union's kernel offsets round-robin (wave w takes ordinals w, w+4, ...),
-// so the longest dependent chain of a block is a quarter of its offsets.  Each
+// the union's kernel offsets by offset index (wave w takes f = w, w+4, ...),
+// so the longest dependent chain of a block is about a quarter of its offsets
+// and each row's summation order does not depend on its tile's other rows.  Each
 // wave accumulates all 64x64 outputs of its offsets in MFMA registers:
 // gathered rows and weight rows are loaded straight into operand registers
 // (16 B per lane) with raw buffer loads whose out-of-range offset reads zero,
@@ -91,16 +92,11 @@ __global__ __launch_bounds__(256) void rowgather_kernel(RowGatherArgs p) {
     for (int s = 0; s < 4; ++s) sm[s] = __builtin_bitreverse32(sm[s]) >> (32 - p.kv);
   }
   const uint32_t uni = (sm[0] | sm[1] | sm[2] | sm[3]);
-  // this wave's share: union ordinals congruent to the wave index mod 4
-  uint32_t mine = 0;
-  {
-    uint32_t r = uni;
-    for (int o = 0; r; ++o) {
-      const uint32_t low = r & (0u - r);
-      if ((o & 3) == wave) mine |= low;
-      r ^= low;
-    }
-  }
+  // This wave's share: offsets f with f % 4 == wave.  The split depends on the
+  // offset index only, never on which offsets the tile's union holds, so every
+  // row's partial sums (and hence its bits) are the same whatever rows share
+  // its tile: a cloud gives identical rows alone and inside a batch.
+  const uint32_t mine = uni & (0x11111111u << wave);
 
   int cols[4];
 #pragma unroll
@@ -225,10 +221,12 @@ __global__ __launch_bounds__(256) void rowgather_kernel(RowGatherArgs p) {
 }
 
 template <typename T, typename OutT>
+// block_n (the algo's tile_shape[1]) caps the weight-stationary kernel's
+// column slice (16, 32 or 64); the block kernel always uses 64 columns.
 static hipError_t launch_rg(const RowGatherArgs& a, int block_n, hipStream_t s) {
-  if (block_n != 64) return hipErrorInvalidValue;
+  if (block_n != 16 && block_n != 32 && block_n != 64) return hipErrorInvalidValue;
   bool used = false;
-  const hipError_t e = launch_ws<T, OutT>(a, s, &used);
+  const hipError_t e = launch_ws<T, OutT>(a, block_n, s, &used);
   if (used || e != hipSuccess) return e;
   dim3 grid((a.m + kRgRows - 1) / kRgRows, (a.nout + 63) / 64, 1);
   dim3 block(256, 1, 1);

@@ -332,10 +332,12 @@ static hipError_t launch_ws_kc(const RowGatherArgs& a, int ncol, hipStream_t s) 
 }
 
 // Launches the weight-stationary kernel when it applies; *used = false otherwise.
+// The column slice is the largest of 16 / 32 / 64 that fits the LDS budget and
+// does not exceed max_ncol.
 template <typename T, typename OutT>
-static hipError_t launch_ws(const RowGatherArgs& a, hipStream_t s, bool* used) {
+static hipError_t launch_ws(const RowGatherArgs& a, int max_ncol, hipStream_t s, bool* used) {
   *used = false;
-  int ncol = ws_ncol<T>(a.kv, a.kred);
+  int ncol = std::min(ws_ncol<T>(a.kv, a.kred), max_ncol);
   if (ncol == 48) ncol = 32;
   if (ncol == 0 || a.pair_col == nullptr) return hipSuccess;
   const int kc = a.kred / Mma<T>::kChunk;

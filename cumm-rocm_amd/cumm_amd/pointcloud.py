@@ -145,15 +145,26 @@ def subm_pairs(coords: np.ndarray, spatial_shape: Sequence[int],
 
 def strided_out_coords(coords: np.ndarray, spatial_shape: Sequence[int],
                        ksize: Sequence[int] = (3, 3, 3), stride: int = 2,
-                       padding: int = 1) -> Tuple[np.ndarray, Tuple[int, int, int]]:
-    """Output voxels of a regular strided sparse conv: every output position whose
-    window ``o*stride - padding + k`` holds at least one active input.  Returns the
-    sorted unique output coords and the output spatial shape
-    ``(S + 2*padding - k) // stride + 1``."""
+                       padding: int = 1, rule: str = "reach") -> Tuple[np.ndarray, Tuple[int, int, int]]:
+    """Output voxels of a regular strided sparse conv, sorted by (b, z, y, x).
+
+    ``rule="reach"``: every output position whose window
+    ``o*stride - padding + k`` holds at least one active input (the active set
+    dilates).  ``rule="down"``: the 2x-downsampled set ``unique(floor(c /
+    stride))`` (each input's own cell; with padding 1 and k = 3 every input lies
+    in its cell's window).  The output spatial shape is
+    ``(S + 2*padding - k) // stride + 1`` either way."""
     shp = np.asarray(spatial_shape, dtype=np.int64)
     ks = np.asarray(ksize, dtype=np.int64)
     out_shape = tuple(int(v) for v in (shp + 2 * padding - ks) // stride + 1)
     c = coords.astype(np.int64)
+    if rule == "down":
+        oc = np.concatenate([c[:, :1], c[:, 1:] // stride], axis=1)
+        oc[:, 1:] = np.minimum(oc[:, 1:], np.asarray(out_shape) - 1)
+        _, first = np.unique(_keys(oc, out_shape), return_index=True)
+        return oc[first].astype(np.int32), out_shape
+    if rule != "reach":
+        raise ValueError(f"unknown output rule {rule!r}")
     cands = []
     for o in kernel_offsets(ksize):
         num = c[:, 1:] + padding - o
@@ -278,8 +289,8 @@ class StridedProblem:
 
 def build_strided(in_coords: np.ndarray, in_shape: Sequence[int],
                   ksize: Sequence[int] = (3, 3, 3), stride: int = 2,
-                  padding: int = 1) -> StridedProblem:
-    out_coords, out_shape = strided_out_coords(in_coords, in_shape, ksize, stride, padding)
+                  padding: int = 1, rule: str = "reach") -> StridedProblem:
+    out_coords, out_shape = strided_out_coords(in_coords, in_shape, ksize, stride, padding, rule)
     pf, pb = strided_pairs(in_coords, in_shape, out_coords, ksize, stride, padding)
     mf, af = sort_masks(row_masks(pf))
     mb, ab = sort_masks(row_masks(pb))
