@@ -27,6 +27,8 @@ SYMBOLS = [
     "cs_gemm_workspace_size", "cs_get_all_conv_algo_desp", "cs_get_all_algo_desp",
     "cs_gemm_algo_name", "cs_conv_algo_name", "cs_device_synchronize",
     "cs_stream_synchronize", "cs_last_error",
+    "cs_pairs_subm_workspace", "cs_pairs_subm", "cs_pairs_strided_out_workspace", "cs_pairs_strided_out",
+    "cs_pairs_strided_workspace", "cs_pairs_strided", "cs_pairs_last_error",
 ]
 
 
@@ -123,6 +125,17 @@ def lib() -> C.CDLL:
     L.cs_stream_synchronize.argtypes = [C.c_uint64]
     L.cs_stream_synchronize.restype = C.c_int
     L.cs_last_error.restype = C.c_char_p
+    P, I64, I32, U64 = C.c_void_p, C.c_int64, C.c_int32, C.c_uint64
+    L.cs_pairs_subm_workspace.argtypes = [I64, C.POINTER(I64)]
+    L.cs_pairs_subm.argtypes = [P, I64, P, P, P, P, P, P, I64, U64]
+    L.cs_pairs_strided_out_workspace.argtypes = [I64, I32, C.POINTER(I64)]
+    L.cs_pairs_strided_out.argtypes = [P, I64, P, I32, P, I64, C.POINTER(I64), P, I64, U64]
+    L.cs_pairs_strided_workspace.argtypes = [I64, I64, C.POINTER(I64)]
+    L.cs_pairs_strided.argtypes = [P, I64, P, P, I64, P, P, P, P, P, P, P, I64, U64]
+    for name in ("cs_pairs_subm_workspace", "cs_pairs_subm", "cs_pairs_strided_out_workspace",
+                 "cs_pairs_strided_out", "cs_pairs_strided_workspace", "cs_pairs_strided"):
+        getattr(L, name).restype = C.c_int
+    L.cs_pairs_last_error.restype = C.c_char_p
     if L.cs_abi_version() != ABI_VERSION:
         raise RuntimeError("libcumm_amd.so ABI version mismatch")
     _lib = L

@@ -98,7 +98,10 @@ def build_stack_tables(kinds: Sequence[str], coords: np.ndarray, shape: Sequence
 
 
 # ----------------------------------------------------------------- device stack
-def _dev_idx(a: np.ndarray, dev) -> torch.Tensor:
+def _dev_idx(a, dev) -> torch.Tensor:
+    """int32 device copy of a host table (device tables pass through)."""
+    if isinstance(a, torch.Tensor):
+        return a.to(dev).contiguous()
     a = np.ascontiguousarray(a)
     if a.dtype == np.uint32:
         a = a.view(np.int32)
@@ -107,7 +110,9 @@ def _dev_idx(a: np.ndarray, dev) -> torch.Tensor:
 
 class DeviceLayer:
     """One conv layer's device-resident tables, weights, buffers and prebuilt
-    params (forward, dgrad, wgrad) for one dtype."""
+    params (forward, dgrad, wgrad) for one dtype.  ``t`` is a host
+    ``LayerTables`` (copied to the device) or a ``pairs.DeviceTables`` built on
+    the device (used in place)."""
 
     def __init__(self, t: LayerTables, C: int, K: int, dt: str, W: np.ndarray, stream: int, dev="cuda"):
         self.t, self.C, self.K, self.dt = t, C, K, dt

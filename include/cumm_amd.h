@@ -148,6 +148,37 @@ int cs_stream_synchronize(uint64_t stream);
 /* Message of the last failing call on this thread ("" if none). */
 const char* cs_last_error(void);
 
+/* ---- Pair tables, masks and argsort on the device (3x3x3 kernels).
+ * These replace what the caller of implicit_gemm2 builds before each layer:
+ * spconv's get_indice_pairs / generate_subm_conv_inds + sort_1d_by_key_split
+ * (called at cumm/conv/main_real_spconv.py:91-145), over tensorview's GPU
+ * linear-probing hash table (include/tensorview/hash/linear.cu.h:111-430).
+ * Coordinates are int32 rows (b, z, y, x); spatial shapes are (z, y, x).
+ * Outputs: pair tables [27][N] int32 (-1 = none), masks uint32 sorted stably
+ * with their argsort.  All buffers are device memory; work is queued on
+ * `stream`.  Return 0 / 1 (bad argument) / 2 (device error), message in
+ * cs_pairs_last_error(). */
+int cs_pairs_subm_workspace(int64_t n, int64_t* nbytes);
+int cs_pairs_subm(const int32_t* coords, int64_t n, const int32_t* spatial_shape, int32_t* pair_fwd,
+                  int32_t* pair_bwd, uint32_t* mask_sorted, int32_t* argsort, void* workspace,
+                  int64_t ws_bytes, uint64_t stream);
+/* Output coordinates of a stride-2, padding-1 layer, sorted by key: rule 0 =
+ * unique(floor(c / 2)), rule 1 = every output whose window reaches an input.
+ * Synchronises `stream` to return the count in *n_out (<= cap rows written). */
+int cs_pairs_strided_out_workspace(int64_t n, int32_t rule, int64_t* nbytes);
+int cs_pairs_strided_out(const int32_t* coords, int64_t n, const int32_t* in_shape, int32_t rule,
+                         int32_t* out_coords, int64_t cap, int64_t* n_out, void* workspace, int64_t ws_bytes,
+                         uint64_t stream);
+/* Strided pair tables: pair_fwd [27][n_out] (input rows), pair_bwd [27][n_in]
+ * (output rows), forward masks over the outputs and backward masks over the
+ * inputs, each sorted with its argsort. */
+int cs_pairs_strided_workspace(int64_t n_in, int64_t n_out, int64_t* nbytes);
+int cs_pairs_strided(const int32_t* in_coords, int64_t n_in, const int32_t* in_shape, const int32_t* out_coords,
+                     int64_t n_out, int32_t* pair_fwd, int32_t* pair_bwd, uint32_t* mask_fwd_sorted,
+                     int32_t* argsort_fwd, uint32_t* mask_bwd_sorted, int32_t* argsort_bwd, void* workspace,
+                     int64_t ws_bytes, uint64_t stream);
+const char* cs_pairs_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,90 @@
+"""Device pair / mask / argsort generation (csrc/cs_pairs.hip) against the host
+builder (cumm_amd.pointcloud, itself pinned by brute force in
+test_oracle.py) and the golden tables: everything int32 / uint32 bit-exact."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+from cumm_amd import batch as B
+from cumm_amd import pairs
+from cumm_amd import pointcloud as pc
+
+pytestmark = pytest.mark.gpu
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+def _np(t):
+    return t.cpu().numpy()
+
+
+def _coords(sizes, grid, seed):
+    return np.concatenate([pc.plane_scene(s, grid=grid, seed=seed + b, batch_index=b) for b, s in enumerate(sizes)])
+
+
+@pytest.mark.parametrize("sizes,grid", [([1], (4, 8, 8)), ([300], (6, 20, 20)), ([5000, 1200, 800], (16, 96, 96)),
+                                        ([20000], pc.GRID_KITTI)])
+def test_subm_tables_match_host(sizes, grid):
+    print(f"[pairs] subm {sizes} {grid}", flush=True)
+    c = _coords(sizes, grid, 3)
+    host = pc.build_subm(c, grid)
+    dev = pairs.subm_tables(torch.from_numpy(c).cuda(), grid)
+    torch.cuda.synchronize()
+    assert np.array_equal(_np(dev.pair_fwd), host.pair_fwd)
+    assert np.array_equal(_np(dev.pair_bwd), host.pair_bwd)
+    assert np.array_equal(_np(dev.mask_fwd).view(np.uint32), host.mask_sorted)
+    assert np.array_equal(_np(dev.argsort_fwd), host.argsort)
+
+
+@pytest.mark.parametrize("rule", ["down", "reach"])
+@pytest.mark.parametrize("sizes,grid", [([1], (4, 8, 8)), ([300], (6, 20, 20)), ([5000, 1200, 800], (16, 96, 96)),
+                                        ([20000], pc.GRID_KITTI)])
+def test_strided_tables_match_host(sizes, grid, rule):
+    print(f"[pairs] strided {rule} {sizes} {grid}", flush=True)
+    c = _coords(sizes, grid, 5)
+    host = pc.build_strided(c, grid, rule=rule)
+    dev = pairs.strided_tables(torch.from_numpy(c).cuda(), grid, rule=rule)
+    torch.cuda.synchronize()
+    assert dev.out_shape == tuple(host.out_shape)
+    assert np.array_equal(_np(dev.out_coords), host.out_coords)
+    assert np.array_equal(_np(dev.pair_fwd), host.pair_fwd)
+    assert np.array_equal(_np(dev.pair_bwd), host.pair_bwd)
+    assert np.array_equal(_np(dev.mask_fwd).view(np.uint32), host.mask_fwd_sorted)
+    assert np.array_equal(_np(dev.argsort_fwd), host.argsort_fwd)
+    assert np.array_equal(_np(dev.mask_bwd).view(np.uint32), host.mask_bwd_sorted)
+    assert np.array_equal(_np(dev.argsort_bwd), host.argsort_bwd)
+
+
+def test_golden_tables_on_device():
+    g = np.load(GOLD / "subm_tiny.npz")
+    dev = pairs.subm_tables(torch.from_numpy(g["coords"]).cuda(), (6, 14, 14))
+    assert np.array_equal(_np(dev.pair_fwd), g["pair_fwd"])
+    assert np.array_equal(_np(dev.argsort_fwd), g["argsort"])
+    g = np.load(GOLD / "strided_tiny.npz")
+    dev = pairs.strided_tables(torch.from_numpy(g["in_coords"]).cuda(), (6, 14, 14), rule="reach")
+    assert np.array_equal(_np(dev.out_coords), g["out_coords"])
+    assert np.array_equal(_np(dev.pair_fwd), g["pair_fwd"])
+    assert np.array_equal(_np(dev.pair_bwd), g["pair_bwd"])
+
+
+def test_device_tables_drive_a_stack():
+    """A stack over device-built tables gives the same bits as over host tables."""
+    grid = (16, 96, 96)
+    c = _coords([3000, 900], grid, 11)
+    ch = (16, 32, 64)
+    rng = np.random.default_rng(2)
+    Ws = [rng.normal(0, 0.1, (ch[i + 1], 27, ch[i])).astype(np.float32) for i in range(2)]
+    host = B.build_stack_tables(("strided", "subm"), c, grid)
+    dev = pairs.stack_tables(("strided", "subm"), torch.from_numpy(c).cuda(), grid)
+    X = rng.standard_normal((c.shape[0], ch[0])).astype(np.float32)
+    dO = rng.standard_normal((host[-1].n_out, ch[-1])).astype(np.float32)
+    outs = []
+    for tabs in (host, dev):
+        st = B.ConvStack(tabs, ch, "f16", Ws, torch.cuda.current_stream().cuda_stream)
+        st.bind(torch.from_numpy(X).cuda().half(), torch.from_numpy(dO).cuda().half())
+        st.step()
+        torch.cuda.synchronize()
+        outs.append([st.layers[-1].out.clone(), st.layers[1].din.clone(), st.layers[0].dw.clone()])
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)
