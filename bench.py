@@ -46,9 +46,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", choices=("c2", "c3", "c4"), default="c2",
+    ap.add_argument("--config", choices=("c2", "c3", "c4", "c5"), default="c2",
                     help="c2: 120k voxels 64->64 (default); c3: precision form, 200k voxels 128->128; "
-                         "c4: 32 clouds, strided 32->64 + subm 64->128, grouped launches under a HIP graph")
+                         "c4: 32 clouds, strided 32->64 + subm 64->128, grouped launches under a HIP graph; "
+                         "c5: serving, Poisson 20-200 frames/s, 4-layer BF16 stack")
+    ap.add_argument("--rates", default="20,50,100,200", help="c5 arrival rates (frames/s)")
+    ap.add_argument("--seconds", type=float, default=3.0, help="c5 traffic per rate")
     ap.add_argument("--voxels", type=int, default=None)
     ap.add_argument("--channels", type=int, default=None)
     ap.add_argument("--seed", type=int, default=0)
@@ -59,7 +62,7 @@ def parse():
     ap.add_argument("--copy-bytes", type=int, default=0)
     args = ap.parse_args()
     dflt = {"c2": (120000, 64, "f32,f16,bf16"), "c3": (200000, 128, "f32,bf16,f16,e4m3"),
-            "c4": (0, 0, "f16")}[args.config]
+            "c4": (0, 0, "f16"), "c5": (0, 0, "bf16")}[args.config]
     args.voxels = args.voxels or dflt[0]
     args.channels = args.channels or dflt[1]
     args.dtypes = args.dtypes or dflt[2]
@@ -408,6 +411,77 @@ def run_batched(args):
     print(json.dumps(line))
 
 
+def _frame_worker(job):
+    from cumm_amd import pointcloud as pc
+
+    size, seed = job
+    return pc.plane_scene(size, seed=seed)
+
+
+def run_serving(args):
+    """Serving form: Poisson arrivals at each rate, frames of 20k-200k voxels
+    (a pool of 16 scenes generated as for the single-layer workload), 4-layer
+    stack subm 16->32, strided 32->64, subm 64->64, strided 64->128.  The BF16
+    server batches continuously on two streams under an HBM budget; the plain
+    FP32 per-call path serves the same trace one frame at a time."""
+    from concurrent.futures import ProcessPoolExecutor
+
+    from cumm_amd import batch as B
+
+    t0 = time.perf_counter()
+    sizes = B.cloud_sizes(16, 20000, 200000, args.seed + 5)
+    with ProcessPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        pool = list(ex.map(_frame_worker, [(s, args.seed + 31 * i) for i, s in enumerate(sizes)]))
+    log(f"c5: frame pool {sizes} generated in {time.perf_counter() - t0:.1f}s")
+    import torch
+    from cumm_amd.serve import Frame, SparseConvServer
+
+    kinds = ("subm", "strided", "subm", "strided")
+    chans = (16, 32, 64, 64, 128)
+    rng = np.random.default_rng(args.seed + 2)
+    Ws = [rng.normal(0.0, 1.0 / np.sqrt(27 * chans[i]), size=(chans[i + 1], 27, chans[i])).astype(np.float32)
+          for i in range(4)]
+    feats = [rng.standard_normal((c.shape[0], chans[0])).astype(np.float32) for c in pool]
+
+    def trace(rate, seed):
+        r = np.random.default_rng(seed)
+        n = max(20, int(rate * args.seconds))
+        idx = r.integers(0, len(pool), n)
+        arr = np.cumsum(r.exponential(1.0 / rate, n))
+        return [Frame(i, pool[j], feats[j]) for i, j in enumerate(idx)], arr
+
+    forms = {"bf16_continuous": dict(dt="bf16", max_batch=16, n_streams=2),
+             "f32_per_call": dict(dt="f32", max_batch=1, n_streams=1)}
+    res = {}
+    for name, kw in forms.items():
+        srv = SparseConvServer(kinds, chans, Ws, keep_outputs=False, hbm_budget=16 << 30, **kw)
+        wf, wa = trace(1000.0, 1)
+        srv.run_trace(wf[:6], wa[:6] - wa[0])  # warm-up: first use of every kernel
+        torch.cuda.synchronize()
+        for rate in [float(v) for v in args.rates.split(",")]:
+            fr, arr = trace(rate, int(rate) + args.seed)
+            st = srv.run_trace(fr, arr)
+            res.setdefault(name, {})[str(int(rate))] = st
+            log(f"c5 {name} {rate:.0f} fps: p50 {st['p50_ms']:.2f} ms  p99 {st['p99_ms']:.2f} ms  "
+                f"{st['frames_per_s']:.1f} frames/s  mean batch {st['mean_batch']:.2f}")
+        del srv
+        torch.cuda.synchronize()
+    top = str(int(float(args.rates.split(",")[-1])))
+    main = res["bf16_continuous"][top]
+    line = {
+        "metric": "sparse-conv TFLOP/s + % MFMA peak per dtype vs plain FP32; serving p50/p99 ms",
+        "value": main["p99_ms"], "unit": "ms (p99 latency)", "n_gpus": 1, "steps": main["frames"], "warmup": 6,
+        "ms_per_step": main["mean_ms"], "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16", "data": "synthetic",
+        "config": {"workload": "serving: Poisson arrivals, frames of 20k-200k voxels, 4-layer stack subm 16->32, "
+                               "strided 32->64, subm 64->64, strided 64->128 (unique(floor(c/2)) outputs)",
+                   "rates_fps": [float(v) for v in args.rates.split(",")], "seconds_per_rate": args.seconds,
+                   "frame_pool_voxels": sizes, "hbm_budget_bytes": 16 << 30},
+        "serving": res,
+    }
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -415,6 +489,8 @@ def main():
         sys.exit(2)
     if args.config == "c4":
         return run_batched(args)
+    if args.config == "c5":
+        return run_serving(args)
     import torch
     from cumm_amd import pointcloud as pc
 

@@ -114,8 +114,8 @@ class DeviceLayer:
     ``LayerTables`` (copied to the device) or a ``pairs.DeviceTables`` built on
     the device (used in place)."""
 
-    def __init__(self, t: LayerTables, C: int, K: int, dt: str, W: np.ndarray, stream: int, dev="cuda"):
-        self.t, self.C, self.K, self.dt = t, C, K, dt
+    def __init__(self, t: LayerTables, C: int, K: int, dt: str, W, stream: int, dev="cuda", train: bool = True):
+        self.t, self.C, self.K, self.dt, self.train = t, C, K, dt, train
         td = TORCH_DT[dt]
         kv = t.pair_fwd.shape[0]
         self.kv = kv
@@ -125,11 +125,17 @@ class DeviceLayer:
         self.af = _dev_idx(t.argsort_fwd, dev)
         self.mb = _dev_idx(t.mask_bwd, dev) if t.mask_bwd is not None else None
         self.ab = _dev_idx(t.argsort_bwd, dev) if t.argsort_bwd is not None else None
-        self.w = torch.from_numpy(np.ascontiguousarray(W, dtype=np.float32)).to(dev, td)
-        self.dw = torch.zeros((K, kv, C), dtype=td, device=dev)
-        self.out = torch.zeros((t.n_out, K), dtype=td, device=dev)
-        self.dout = torch.zeros((t.n_out, K), dtype=td, device=dev)
-        self.din = torch.zeros((t.n_in, C), dtype=td, device=dev)
+        if isinstance(W, torch.Tensor):  # device weights already in the layer dtype are shared
+            self.w = W if (W.dtype == td and W.device.type == "cuda") else W.to(dev, td)
+        else:
+            self.w = torch.from_numpy(np.ascontiguousarray(W, dtype=np.float32)).to(dev, td)
+        self.out = torch.empty((t.n_out, K), dtype=td, device=dev)
+        if train:
+            self.dw = torch.zeros((K, kv, C), dtype=td, device=dev)
+            self.dout = torch.zeros((t.n_out, K), dtype=td, device=dev)
+            self.din = torch.zeros((t.n_in, C), dtype=td, device=dev)
+        else:
+            self.dw = self.dout = self.din = None
         self.mask_out = torch.zeros(((t.n_out + 15) // 16 + 1,), dtype=torch.int32, device=dev)
         self.stream = stream
         self.x = None
@@ -170,7 +176,7 @@ class DeviceLayer:
                 p.workspace = tv.from_torch(torch.empty((ws,), dtype=torch.uint8, device=self.out.device))
             return p
 
-        self.params = {"fwd": mk(0), "dgrad": mk(1), "wgrad": mk(2)}
+        self.params = {"fwd": mk(0), "dgrad": mk(1), "wgrad": mk(2)} if self.train else {"fwd": mk(0)}
 
     def run(self, op: str) -> None:
         ConvMainUnitTest.implicit_gemm2(self.params[op])
@@ -188,16 +194,19 @@ class ConvStack:
 
     def __init__(self, tables: List[LayerTables], channels: Sequence[int], dt: str, weights: Sequence[np.ndarray],
                  stream: int, train: bool = True):
-        self.layers = [DeviceLayer(t, channels[i], channels[i + 1], dt, weights[i], stream)
+        self.layers = [DeviceLayer(t, channels[i], channels[i + 1], dt, weights[i], stream, train=train)
                        for i, t in enumerate(tables)]
         self.train = train
         self.graph = None
 
-    def bind(self, x: torch.Tensor, dout: torch.Tensor) -> None:
+    def bind(self, x: torch.Tensor, dout: Optional[torch.Tensor] = None) -> None:
         L = self.layers
         # layer i reads layer i-1's output; its output gradient is what layer
         # i+1's dgrad writes (the last layer's comes from outside)
         for i, l in enumerate(L):
+            if not self.train:
+                l.bind(x if i == 0 else L[i - 1].out)
+                continue
             l.bind(x if i == 0 else L[i - 1].out, dout if i == len(L) - 1 else L[i + 1].din)
 
     def ops(self) -> List[Tuple[DeviceLayer, str]]:

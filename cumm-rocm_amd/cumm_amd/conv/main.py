@@ -8,6 +8,8 @@ gfx950 kernels in libcumm_amd.so through the C ABI of include/cumm_amd.h.
 """
 from __future__ import annotations
 
+import copy
+
 import ctypes as C
 from typing import List
 
@@ -67,22 +69,28 @@ class ConvMainUnitTest:
         return table.get(op, [])
 
 
+_CATALOGUE: List[ConvAlgoDesp] = []
+
+
 def select_conv_algo(op_type: int, dtype_input: int, dtype_weight: int, dtype_output: int,
                      C: int, K: int) -> ConvAlgoDesp:
     """Pick this build's kernel for a conv layer, the way a caller filters
-    ``get_all_conv_algo_desp()`` by op and dtypes (64 output columns per wave)."""
+    ``get_all_conv_algo_desp()`` by op and dtypes (64 output columns per wave).
+    Returns a fresh copy; the catalogue is read from the library once."""
+    if not _CATALOGUE:
+        _CATALOGUE.extend(ConvMainUnitTest.get_all_conv_algo_desp())
     op = int(op_type)
     bn = 64
     best = None
-    for d in ConvMainUnitTest.get_all_conv_algo_desp():
+    for d in _CATALOGUE:
         if int(d.op_type) != op:
             continue
         if (d.dtype_input, d.dtype_weight, d.dtype_output) != (dtype_input, dtype_weight, dtype_output):
             continue
         if op == 2 or d.tile_shape[1] == bn:
-            return d
+            return copy.deepcopy(d)
         best = best or d
     if best is None:
         raise ValueError(f"no kernel for op {op} with dtypes input={tv.dtype_short_str(dtype_input)} "
                          f"weight={tv.dtype_short_str(dtype_weight)} output={tv.dtype_short_str(dtype_output)}")
-    return best
+    return copy.deepcopy(best)

@@ -1,0 +1,212 @@
+"""Continuous-batching serving loop for sparse-conv stacks.
+
+Requests are voxel frames (int32 coordinates + float features on the host)
+that arrive over time.  The server
+
+* admits a waiting frame when a slot is free and its estimated device memory
+  fits the HBM budget still unreserved (frames larger than the whole budget
+  are refused at submission);
+* forms a batch from every admitted frame not yet started (up to
+  ``max_batch`` frames) as soon as a stream is idle, so frames that arrive
+  while others are on the GPU join the next batch instead of waiting for the
+  queue to drain;
+* runs each batch on its own HIP stream from a small pool: upload, device pair
+  tables for every layer (cumm_amd.pairs), then one implicit_gemm2 launch per
+  layer over all frames of the batch; features, tables and activations stay
+  on the device;
+* completes frames by polling the batch's event, releasing their memory
+  reservation and slot, and records latency = completion - arrival.
+
+A frame's output rows are bit-identical alone and inside any batch: pair
+tables never connect two frames (the frame index is coordinate column 0), and
+the forward kernels sum each row's offsets in a fixed order whatever rows
+share its tile.
+"""
+from __future__ import annotations
+
+import collections
+import dataclasses
+import time
+from typing import Deque, Dict, List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import batch as B
+from . import pairs
+
+GRID = (41, 1600, 1408)
+
+
+@dataclasses.dataclass
+class Frame:
+    fid: int
+    coords: np.ndarray       # [N, 4] int32, column 0 ignored (set per batch)
+    feats: np.ndarray        # [N, C0] float32
+    arrival: float = 0.0     # seconds on the server clock
+    start: float = 0.0
+    done: float = 0.0
+    batch_size: int = 0
+    out: Optional[torch.Tensor] = None
+    out_coords: Optional[torch.Tensor] = None
+
+
+@dataclasses.dataclass
+class _Batch:
+    frames: List[Frame]
+    stream: torch.cuda.Stream
+    event: torch.cuda.Event
+    keep: list               # device objects alive until the batch completes
+    reserved: int
+
+
+class SparseConvServer:
+    def __init__(self, kinds: Sequence[str], channels: Sequence[int], weights: Sequence[np.ndarray], dt: str = "bf16",
+                 hbm_budget: int = 8 << 30, max_inflight: int = 64, max_batch: int = 16, n_streams: int = 2,
+                 grid=GRID, keep_outputs: bool = True):
+        assert len(kinds) + 1 == len(channels) == len(weights) + 1
+        self.kinds, self.channels, self.dt = tuple(kinds), tuple(channels), dt
+        self.weights = [torch.from_numpy(np.ascontiguousarray(w, dtype=np.float32)).cuda().to(B.TORCH_DT[dt])
+                        for w in weights]
+        self.budget = int(hbm_budget)
+        self.reserved = 0
+        self.max_inflight, self.max_batch = max_inflight, max_batch
+        self.streams = [torch.cuda.Stream() for _ in range(n_streams)]
+        self.busy: Dict[int, Optional[_Batch]] = {i: None for i in range(n_streams)}
+        self.grid = grid
+        self.keep_outputs = keep_outputs
+        self.waiting: Deque[Frame] = collections.deque()
+        self.admitted: Deque[Frame] = collections.deque()
+        self.inflight = 0
+        self.completed: List[Frame] = []
+        self.t0 = time.perf_counter()
+
+    # ------------------------------------------------------------ memory model
+    def frame_bytes(self, n: int) -> int:
+        """Upper estimate of one frame's device footprint: coordinates,
+        features and activations of every layer, pair tables and masks of
+        every layer (a strided layer's outputs are bounded by its inputs for
+        the downsampling rule), pair-builder scratch and conv workspaces."""
+        es = 2 if self.dt in ("f16", "bf16") else 4
+        per_row = 16 + 4 * self.channels[0] + es * self.channels[0]
+        for c in self.channels[1:]:
+            per_row += es * c + 16
+        per_row += len(self.kinds) * (27 * 4 * 2 + 8 * 4 + 64)
+        return int(per_row * n * 1.25) + (4 << 20)
+
+    def now(self) -> float:
+        return time.perf_counter() - self.t0
+
+    # ------------------------------------------------------------ requests
+    def submit(self, f: Frame) -> None:
+        if self.frame_bytes(f.coords.shape[0]) > self.budget:
+            raise ValueError(f"frame {f.fid} needs more than the whole HBM budget")
+        self.waiting.append(f)
+
+    def _admit(self) -> None:
+        while self.waiting and self.inflight < self.max_inflight:
+            f = self.waiting[0]
+            need = self.frame_bytes(f.coords.shape[0])
+            if self.reserved + need > self.budget:
+                break
+            self.waiting.popleft()
+            self.reserved += need
+            self.inflight += 1
+            self.admitted.append(f)
+
+    def _idle_stream(self) -> Optional[int]:
+        for i, b in self.busy.items():
+            if b is None:
+                return i
+        return None
+
+    def _launch(self, si: int) -> None:
+        frames = []
+        while self.admitted and len(frames) < self.max_batch:
+            frames.append(self.admitted.popleft())
+        s = self.streams[si]
+        now = self.now()
+        coords = np.concatenate([f.coords for f in frames])
+        off = 0
+        for b, f in enumerate(frames):
+            n = f.coords.shape[0]
+            coords[off:off + n, 0] = b
+            off += n
+            f.start = now
+            f.batch_size = len(frames)
+        feats = np.concatenate([f.feats for f in frames])
+        with torch.cuda.stream(s):
+            c_dev = torch.from_numpy(coords).pin_memory().to("cuda", non_blocking=True)
+            x_dev = torch.from_numpy(feats).pin_memory().to("cuda", non_blocking=True).to(B.TORCH_DT[self.dt])
+            tabs = pairs.stack_tables(self.kinds, c_dev, self.grid, stream=s.cuda_stream)
+            st = B.ConvStack(tabs, self.channels, self.dt, self.weights, s.cuda_stream, train=False)
+            st.bind(x_dev)
+            st.step()
+            ev = torch.cuda.Event()
+            ev.record(s)
+        self.busy[si] = _Batch(frames, s, ev, [c_dev, x_dev, tabs, st], sum(self.frame_bytes(f.coords.shape[0])
+                                                                             for f in frames))
+
+    def _poll(self) -> None:
+        for si, b in self.busy.items():
+            if b is None or not b.event.query():
+                continue
+            t = self.now()
+            st, tabs = b.keep[3], b.keep[2]
+            out = st.layers[-1].out
+            oc = tabs[-1].out_coords
+            if self.keep_outputs:
+                # output rows of frame k are contiguous: keys sort frame-major
+                counts = torch.bincount(oc[:, 0].long(), minlength=len(b.frames)).tolist()
+                offs = np.cumsum([0] + counts)
+                for k, f in enumerate(b.frames):
+                    f.out = out[offs[k]:offs[k + 1]]
+                    f.out_coords = oc[offs[k]:offs[k + 1]]
+            for f in b.frames:
+                f.done = t
+                self.completed.append(f)
+            self.reserved -= b.reserved
+            self.inflight -= len(b.frames)
+            self.busy[si] = None
+
+    def pump(self) -> bool:
+        """One scheduler pass; True if it did anything."""
+        before = (len(self.completed), len(self.admitted), len(self.waiting))
+        self._poll()
+        self._admit()
+        si = self._idle_stream()
+        launched = False
+        if self.admitted and si is not None:
+            self._launch(si)
+            launched = True
+        return launched or before != (len(self.completed), len(self.admitted), len(self.waiting))
+
+    def drain(self) -> None:
+        while self.waiting or self.admitted or any(b is not None for b in self.busy.values()):
+            if not self.pump():
+                time.sleep(5e-5)
+
+    # ------------------------------------------------------------ trace driver
+    def run_trace(self, frames: Sequence[Frame], arrivals: Sequence[float]) -> dict:
+        """Replays ``frames`` arriving at ``arrivals`` (seconds from start);
+        returns latency / throughput statistics."""
+        order = np.argsort(arrivals, kind="stable")
+        self.t0 = time.perf_counter()
+        i = 0
+        n = len(frames)
+        while i < n or self.waiting or self.admitted or any(b is not None for b in self.busy.values()):
+            t = self.now()
+            while i < n and arrivals[order[i]] <= t:
+                f = frames[order[i]]
+                f.arrival = float(arrivals[order[i]])
+                self.submit(f)
+                i += 1
+            if not self.pump():
+                nxt = arrivals[order[i]] - self.now() if i < n else 0.0
+                time.sleep(min(max(nxt, 0.0), 2e-4) if nxt > 0 else 5e-5)
+        lat = np.array([f.done - f.arrival for f in frames])
+        span = max(f.done for f in frames) - min(f.arrival for f in frames)
+        return {"frames": n, "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                "p99_ms": float(np.percentile(lat, 99) * 1e3), "mean_ms": float(lat.mean() * 1e3),
+                "max_ms": float(lat.max() * 1e3), "frames_per_s": n / span if span > 0 else float("inf"),
+                "mean_batch": float(np.mean([f.batch_size for f in frames]))}
