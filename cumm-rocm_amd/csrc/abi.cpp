@@ -428,26 +428,42 @@ int64_t align256(int64_t v) { return (v + 255) / 256 * 256; }
 
 // Pieces per offset of the planned wgrad partition: the caller's split_k_slices
 // when > 1, else a fixed default; depends only on the call's parameters.
-int resolve_splits(int requested) {
-  if (requested > 1) return std::min(requested, cs::kWgradMaxBlocks);
-  return cs::kWgradMaxBlocks;
+int resolve_splits(int requested, int64_t M, int64_t N) {
+  int s = requested > 1 ? std::min(requested, cs::kWgradMaxBlocks) : cs::kWgradMaxBlocks;
+  // one offset's partials are addressed with 32-bit byte offsets
+  const int64_t per = std::max<int64_t>(1, M * N * 4);
+  s = (int)std::min<int64_t>(s, std::max<int64_t>(1, 2147483647LL / per));
+  return s;
 }
 
-// Lower bound of 16-row groups per wgrad piece: pieces of an offset scale with
-// its active groups, at least 2 chunks of 256 rows each.
-constexpr int kWgradMinGroups = 32;
+// Lower bound of pairs per wgrad piece: pieces of an offset scale with its
+// pair count, at least two chunks of 256 rows.
+constexpr int kWgradMinRows = 512;
 
+// wgrad workspace: FP32 partials, then (conv form only) the effective row
+// masks, per-block pair counts, the per-offset pair lists and their totals.
 struct WgradLayout {
-  int64_t partial = 0, glist = 0, gcount = 0, total = 0;
+  int64_t partial = 0, rowmask = 0, bcount = 0, list_a = 0, list_b = 0, total = 0, size = 0;
 };
 
-WgradLayout wgrad_layout(int64_t kv, int64_t M, int64_t N, int64_t nrows, int splits) {
+WgradLayout wgrad_layout(int64_t kv, int64_t M, int64_t N, int64_t nrows, int splits, bool lists) {
   WgradLayout w;
-  const int64_t ngroups = (nrows + 15) / 16;
   w.partial = 0;
-  w.glist = align256(kv * splits * M * N * 4);
-  w.gcount = w.glist + align256(kv * std::max<int64_t>(ngroups, 1) * 4);
-  w.total = w.gcount + align256(kv * 4);
+  int64_t off = align256(kv * splits * M * N * 4);
+  if (lists) {
+    const int64_t nblk = std::max<int64_t>(1, (nrows + 4095) / 4096);
+    w.rowmask = off;
+    off += align256(std::max<int64_t>(nrows, 1) * 4);
+    w.bcount = off;
+    off += align256(kv * nblk * 4);
+    w.list_a = off;
+    off += align256(kv * std::max<int64_t>(nrows, 1) * 4);
+    w.list_b = off;
+    off += align256(kv * std::max<int64_t>(nrows, 1) * 4);
+    w.total = off;
+    off += align256(kv * 4);
+  }
+  w.size = off;
   return w;
 }
 
@@ -575,8 +591,9 @@ ConvPlan plan_conv(const cs_conv_params& p) {
   if (desp.op_type == 1) {
     plan.workspace = align256(kv * C * K * dtype_bytes(dt_w));
   } else if (desp.op_type == 2) {
-    plan.splits = resolve_splits(p.split_k_slices);
-    plan.workspace = wgrad_layout(kv, K, C, N, plan.splits).total;
+    CS_RT(K * C * 4 < kInt32Max, "your data exceed int32 range (wgrad output).");
+    plan.splits = resolve_splits(p.split_k_slices, K, C);
+    plan.workspace = wgrad_layout(kv, K, C, N, plan.splits, true).size;
   }
   return plan;
 }
@@ -656,21 +673,26 @@ void run_conv(const cs_conv_params& p) {
     w.N = (int32_t)plan.C;
     w.kv = (int32_t)plan.kv;
     w.mask_filter = p.mask_filter & (plan.kv >= 32 ? 0xffffffffu : ((1u << plan.kv) - 1u));
-    w.ngroups = (int32_t)((plan.N + 15) / 16);
     w.max_blocks = plan.splits;
-    w.min_groups = kWgradMinGroups;
-    const WgradLayout lay = wgrad_layout(plan.kv, plan.K, plan.C, plan.N, plan.splits);
+    w.min_rows = kWgradMinRows;
+    w.fixed_total = -1;
+    const WgradLayout lay = wgrad_layout(plan.kv, plan.K, plan.C, plan.N, plan.splits, true);
     char* base = static_cast<char*>(ws.ptr);
     w.partial = reinterpret_cast<float*>(base + lay.partial);
-    w.glist = reinterpret_cast<int32_t*>(base + lay.glist);
-    w.gcount = reinterpret_cast<int32_t*>(base + lay.gcount);
+    w.rowmask = reinterpret_cast<uint32_t*>(base + lay.rowmask);
+    w.bcount = reinterpret_cast<int32_t*>(base + lay.bcount);
+    w.list_a = reinterpret_cast<int32_t*>(base + lay.list_a);
+    w.list_b = reinterpret_cast<int32_t*>(base + lay.list_b);
+    w.list_ld = plan.N;
+    w.total = reinterpret_cast<int32_t*>(base + lay.total);
     CS_HIP(cs::launch_wgrad(p.output.dtype, w, s));
     cs::ReduceArgs r;
     std::memset(&r, 0, sizeof(r));
     r.partial = w.partial;
-    r.gcount = w.gcount;
+    r.total = w.total;
+    r.fixed_total = -1;
     r.max_blocks = w.max_blocks;
-    r.min_groups = w.min_groups;
+    r.min_rows = w.min_rows;
     r.kv = (int32_t)plan.kv;
     r.M = (int32_t)plan.K;
     r.N = (int32_t)plan.C;
@@ -783,8 +805,9 @@ GemmPlan plan_gemm(const cs_gemm_params& p) {
            "m, n and row strides must be multiples of ", vec);
     const int64_t tiles = ((g.m + cs::kWgradTile - 1) / cs::kWgradTile) * ((g.n + cs::kWgradTile - 1) / cs::kWgradTile);
     (void)tiles;
-    g.splits = resolve_splits(p.split_k_slices);
-    g.workspace = wgrad_layout(1, g.m, g.n, g.k, g.splits).total;
+    CS_RT(g.m * g.n * 4 < kInt32Max, "your data exceed int32 range (c).");
+    g.splits = resolve_splits(p.split_k_slices, g.m, g.n);
+    g.workspace = wgrad_layout(1, g.m, g.n, g.k, g.splits, false).size;
   } else {
     CS_ARG(g.k % vec == 0, "k must be a multiple of ", vec);
     if (!trans_a) CS_ARG(row_stride(p.a) % vec == 0, "lda must be a multiple of ", vec);
@@ -812,31 +835,29 @@ void run_gemm(const cs_gemm_params& p) {
     w.ldb = row_stride(p.b);
     w.a_bytes = (uint32_t)span_bytes(p.a);
     w.b_bytes = (uint32_t)span_bytes(p.b);
-    w.a_rows = reinterpret_cast<const int32_t*>(p.a_inds.data);
-    w.b_rows = reinterpret_cast<const int32_t*>(p.b_inds.data);
-    w.pairs = nullptr;
-    w.mask = nullptr;
-    w.mask_width = 1;
+    // list form: the index tensors are the pair list (every row counts)
+    w.list_a = const_cast<int32_t*>(reinterpret_cast<const int32_t*>(p.a_inds.data));
+    w.list_b = const_cast<int32_t*>(reinterpret_cast<const int32_t*>(p.b_inds.data));
+    w.list_ld = 0;
+    w.fixed_total = (int32_t)g.k;
     w.nrows = (int32_t)g.k;
     w.M = (int32_t)g.m;
     w.N = (int32_t)g.n;
     w.kv = 1;
     w.mask_filter = 1u;
-    w.ngroups = (int32_t)((g.k + 15) / 16);
     w.max_blocks = g.splits;
-    w.min_groups = kWgradMinGroups;
-    const WgradLayout lay = wgrad_layout(1, g.m, g.n, g.k, g.splits);
+    w.min_rows = kWgradMinRows;
+    const WgradLayout lay = wgrad_layout(1, g.m, g.n, g.k, g.splits, false);
     char* base = static_cast<char*>(ws.ptr);
     w.partial = reinterpret_cast<float*>(base + lay.partial);
-    w.glist = reinterpret_cast<int32_t*>(base + lay.glist);
-    w.gcount = reinterpret_cast<int32_t*>(base + lay.gcount);
     CS_HIP(cs::launch_wgrad(p.a.dtype, w, s));
     cs::ReduceArgs r;
     std::memset(&r, 0, sizeof(r));
     r.partial = w.partial;
-    r.gcount = w.gcount;
+    r.total = nullptr;
+    r.fixed_total = w.fixed_total;
     r.max_blocks = w.max_blocks;
-    r.min_groups = w.min_groups;
+    r.min_rows = w.min_rows;
     r.kv = 1;
     r.M = (int32_t)g.m;
     r.N = (int32_t)g.n;

@@ -40,38 +40,44 @@ struct RowGatherArgs {
 };
 
 // Row-pair outer-product reduction (sparse conv wgrad, ShuffleAB GEMM).
-//   out[f][mm][nn] = sum_r A[ra(r)][mm] * B[rb(r,f)][nn]
-// ra(r) = a_rows[r];  rb(r,f) = pairs ? pairs[f*pair_ld + ra(r)]
-// : (b_rows ? b_rows[r] : r); row r counts for offset f only if rb >= 0 and
-// (mask == null or bit f of mask[r / mask_width]).
-// A plan kernel lists, per offset, the 16-row groups whose mask word has the
-// offset (glist / gcount).  Offset f's list is cut into nb_f <= max_blocks
-// contiguous pieces of at least min_groups groups; each block writes an FP32
-// partial for its piece and a fixed-order tree reduces the pieces
-// (deterministic: the partition depends only on the masks; no atomics).
+//   out[f][mm][nn] = sum over offset f's pairs (i, j) of A[i][mm] * B[j][nn]
+// Conv form (fixed_total < 0): sorted row r, original row i = a_rows[r]
+// (argsort); i holds a pair for f when j = pairs[f*pair_ld + i] >= 0 and bit f
+// of the mask word of r's group (mask[r / mask_width], all ones when mask is
+// null) and of mask_filter are set.  Kernels build rowmask[i], then per
+// offset the list of (i, j) in ascending i (list_a / list_b, total[f]).
+// List form (fixed_total >= 0, kv = 1): list_a / list_b are given (a_inds,
+// b_inds) with fixed_total entries.
+// Offset f's list is cut into nb_f = min(max_blocks, div_up(total_f, min_rows))
+// contiguous pieces; each block writes an FP32 partial for its piece and a
+// fixed-order tree reduces the pieces (deterministic: the partition depends
+// only on the inputs; no atomics).
 struct WgradArgs {
   const void* a; int64_t lda;
   const void* b; int64_t ldb;
   uint32_t a_bytes, b_bytes;  // extents of a and b (< 2^31): buffer-load range
-  const int32_t* a_rows;      // required
-  const int32_t* b_rows;      // required when pairs is null
-  const int32_t* pairs; int64_t pair_ld;  // pairs rows hold nrows entries
+  const int32_t* a_rows;      // conv form: mask argsort
+  const int32_t* pairs; int64_t pair_ld;
   const uint32_t* mask; int32_t mask_width; int32_t num_mask;
   int32_t nrows, M, N, kv;
   uint32_t mask_filter;
-  int32_t ngroups;           // div_up(nrows, 16)
   int32_t max_blocks;        // pieces per offset (grid.z), <= kWgradMaxBlocks
-  int32_t min_groups;        // groups per piece, lower bound
-  int32_t* glist;            // [kv][ngroups]
-  int32_t* gcount;           // [kv]
+  int32_t min_rows;          // pairs per piece, lower bound
+  int32_t fixed_total;       // list form: entries of the given lists; else -1
+  uint32_t* rowmask;         // [nrows]
+  int32_t* bcount;           // [kv][div_up(nrows, 4096)]
+  int32_t* list_a;           // [kv][list_ld] (list form: a_inds, read only)
+  int32_t* list_b;           // (list form: b_inds, read only)
+  int64_t list_ld;
+  int32_t* total;            // [kv]
   float* partial;            // [kv][max_blocks][M][N] FP32
 };
 
 // out[mm*o_ms + f*o_fs + nn] = act(alpha * tree_sum_b partial[f][b][mm][nn] + beta * src)
-// over the nb_f pieces of offset f (from gcount, as in the wgrad kernel);
+// over the nb_f pieces of offset f (from total / fixed_total, as in the wgrad kernel);
 // src = src_is_bias ? src[nn] : src[mm*s_ms + f*s_fs + nn]  (src may alias out)
 struct ReduceArgs {
-  const float* partial; const int32_t* gcount; int32_t max_blocks, min_groups, kv, M, N;
+  const float* partial; const int32_t* total; int32_t fixed_total, max_blocks, min_rows, kv, M, N;
   void* out; int64_t o_ms, o_fs;
   const void* src; int64_t s_ms, s_fs; int32_t src_is_bias;
   uint32_t mask_filter;

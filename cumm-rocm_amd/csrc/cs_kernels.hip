@@ -2,10 +2,13 @@
 //
 // rowgather_kernel  : (cs_rowgather.h, instantiated per input dtype in
 //                     rg_<dtype>.hip) forward / dgrad, ShuffleAC / dense GEMM.
-// wgrad_kernel      : sparse-conv wgrad / ShuffleAB GEMM.  A block stages 64
-//                     gathered rows of both operands into LDS transposed
-//                     (channel-major), so MFMA operands are contiguous along
-//                     the reduction (row) dimension.  FP32 partials per split.
+// wgrad_*          : sparse-conv wgrad / ShuffleAB GEMM.  Per offset, a pair
+//                     list (a row, b row) in original row order is built from
+//                     the pair table and the effective row masks; the main
+//                     kernel stages 64 gathered rows of both operands into LDS
+//                     transposed (channel-major), so MFMA operands are
+//                     contiguous along the reduction (row) dimension, and
+//                     writes an FP32 partial per piece of the list.
 // splitk_reduce     : fixed pairwise tree over the split partials + epilogue.
 #include <climits>
 #include <cstdint>
@@ -42,67 +45,124 @@ template <> struct TrStore<e4m3> {
   }
 };
 
-// pieces of offset f's group list: nb = min(max_blocks, div_up(count, min_groups))
-__device__ __forceinline__ int wgrad_pieces(int count, int max_blocks, int min_groups) {
-  const int nb = (count + min_groups - 1) / min_groups;
+// pieces of offset f's pair list: nb = min(max_blocks, div_up(count, min_rows))
+__device__ __forceinline__ int wgrad_pieces(int count, int max_blocks, int min_rows) {
+  const int nb = (count + min_rows - 1) / min_rows;
   return nb < max_blocks ? nb : max_blocks;
 }
 
-// One block per offset: list the 16-row groups whose reduced mask word has the
-// offset, in row order.  Each thread reads PG consecutive mask words at once
-// (one round trip per 256*PG groups), then a block-wide exclusive scan of the
-// per-thread counts places its groups.
-__global__ __launch_bounds__(256) void wgrad_plan_kernel(WgradArgs p) {
-  constexpr int PG = 32;
-  const int f = blockIdx.x;
+__device__ __forceinline__ int wgrad_total(const WgradArgs& p, int f) {
+  return p.fixed_total >= 0 ? p.fixed_total : p.total[f];
+}
+
+// Effective mask of every original row: rowmask[argsort[r]] = mask word of
+// sorted row r's group & filter.  With it the pair walk below runs in original
+// row order (coalesced pair-table reads, ascending output rows) and skips
+// exactly the rows the sorted walk would skip.
+__global__ __launch_bounds__(256) void wgrad_rowmask_kernel(WgradArgs p) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= p.nrows) return;
+  const int i = p.a_rows[r];
+  uint32_t m = p.mask_filter;
+  if (p.mask != nullptr) {
+    const int mi = (int)((int64_t)r / p.mask_width);
+    m &= mi < p.num_mask ? p.mask[mi] : 0u;
+  }
+  if ((unsigned)i < (unsigned)p.nrows) p.rowmask[i] = m;
+}
+
+// Pair-list construction, two passes over [kv] x [blocks of kWgradListRows
+// rows].  A thread takes rows i0 + t + 256*k (k < 16), so every load and store
+// instruction covers 256 consecutive rows.  Row i holds a pair for offset f
+// when pair[f][i] >= 0 and bit f of rowmask[i] is set.
+constexpr int kWgradListRows = 4096;
+
+// Loads the 16 rows' pair entries and row masks of one thread (unconditional
+// buffer loads; rows past nrows read 0 and are masked by the row test) and
+// returns the bits of the rows that hold a pair for offset f.
+__device__ __forceinline__ uint32_t wgrad_rows(const WgradArgs& p, int f, int i0, int tid, int (&jv)[16]) {
+  const __amdgpu_buffer_rsrc_t rp = make_rsrc(p.pairs + (int64_t)f * p.pair_ld, (uint32_t)p.nrows * 4u);
+  const __amdgpu_buffer_rsrc_t rm = make_rsrc(p.rowmask, (uint32_t)p.nrows * 4u);
+  uint32_t mv[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int i = i0 + tid + 256 * k;
+    const uint32_t off = i < p.nrows ? (uint32_t)i * 4u : kBufOOB;
+    jv[k] = (int)__builtin_amdgcn_raw_buffer_load_b32(rp, off, 0, 0);
+    mv[k] = __builtin_amdgcn_raw_buffer_load_b32(rm, off, 0, 0);
+  }
+  uint32_t bits = 0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int i = i0 + tid + 256 * k;
+    bits |= (uint32_t)(i < p.nrows && jv[k] >= 0 && ((mv[k] >> f) & 1u)) << k;
+  }
+  return bits;
+}
+
+__global__ __launch_bounds__(256) void wgrad_count_kernel(WgradArgs p) {
+  const int f = blockIdx.y;
+  const int i0 = blockIdx.x * kWgradListRows;
+  const int tid = threadIdx.x;
+  __shared__ int wsum[4];
+  int cnt = 0;
+  if ((p.mask_filter >> f) & 1u) {
+    int jv[16];
+    cnt = __popc(wgrad_rows(p, f, i0, tid, jv));
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) cnt += __shfl_xor(cnt, d);
+  if ((tid & 63) == 0) wsum[tid >> 6] = cnt;
+  __syncthreads();
+  if (tid == 0) p.bcount[f * gridDim.x + blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+__global__ __launch_bounds__(256) void wgrad_list_kernel(WgradArgs p) {
+  const int f = blockIdx.y;
+  const int blk = blockIdx.x;
+  const int nblk = gridDim.x;
+  const int i0 = blk * kWgradListRows;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  __shared__ int wave_sum[4];
+  __shared__ int wsum[16][4];
+  __shared__ int sbase;
   if (!((p.mask_filter >> f) & 1u)) {
-    if (tid == 0) p.gcount[f] = 0;
+    if (blk == 0 && tid == 0) p.total[f] = 0;
     return;
   }
-  const __amdgpu_buffer_rsrc_t mrs = make_rsrc(p.mask, p.mask ? (uint32_t)p.num_mask * 4u : 0u);
-  int base = 0;
-  for (int c0 = 0; c0 < p.ngroups; c0 += 256 * PG) {
-    const int g0 = c0 + tid * PG;
-    // unconditional buffer loads (out-of-range words read as 0), all in flight at once
-    uint32_t w[PG];
-    if (p.mask) {
+  int jv[16];
+  const uint32_t bits = wgrad_rows(p, f, i0, tid, jv);
+  // base = valid pairs of this offset in earlier blocks
+  int part = 0;
+  for (int b = tid; b < blk; b += 256) part += p.bcount[f * nblk + b];
 #pragma unroll
-      for (int i = 0; i < PG; ++i) {
-        const uint32_t wi = (uint32_t)(g0 + i) * 16u / (uint32_t)p.mask_width;
-        w[i] = __builtin_amdgcn_raw_buffer_load_b32(mrs, g0 + i < p.ngroups ? wi * 4u : kBufOOB, 0, 0);
-      }
-    } else {
+  for (int d = 32; d >= 1; d >>= 1) part += __shfl_xor(part, d);
+  // per-wave counts of every 256-row step k
 #pragma unroll
-      for (int i = 0; i < PG; ++i) w[i] = g0 + i < p.ngroups ? 0xffffffffu : 0u;
-    }
-    uint32_t bits = 0u;
-#pragma unroll
-    for (int i = 0; i < PG; ++i) bits |= ((w[i] >> f) & 1u) << i;
-    const int cnt = __popc(bits);
-    // inclusive scan of cnt over the wave
-    int incl = cnt;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const int v = __shfl_up(incl, d);
-      if (lane >= d) incl += v;
-    }
-    if (lane == 63) wave_sum[wave] = incl;
-    __syncthreads();
-    int off = base + incl - cnt;
-    for (int wv = 0; wv < wave; ++wv) off += wave_sum[wv];
-    int* out = p.glist + (int64_t)f * p.ngroups + off;
-    int k = 0;
-    while (bits) {
-      const int i = __builtin_ctz(bits);
-      bits &= bits - 1u;
-      out[k++] = g0 + i;
-    }
-    base += wave_sum[0] + wave_sum[1] + wave_sum[2] + wave_sum[3];
-    __syncthreads();
+  for (int k = 0; k < 16; ++k) {
+    const uint64_t bal = __ballot((bits >> k) & 1u);
+    if (lane == 0) wsum[k][wave] = __popcll(bal);
   }
-  if (tid == 0) p.gcount[f] = base;
+  __shared__ int wpart[4];
+  if (lane == 0) wpart[wave] = part;
+  __syncthreads();
+  if (tid == 0) sbase = wpart[0] + wpart[1] + wpart[2] + wpart[3];
+  __syncthreads();
+  int base = sbase;
+  int32_t* la = p.list_a + (int64_t)f * p.list_ld;
+  int32_t* lb = p.list_b + (int64_t)f * p.list_ld;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const bool v = (bits >> k) & 1u;
+    const uint64_t bal = __ballot(v);
+    int off = base + __popcll(bal & ((1ull << lane) - 1ull));
+    for (int w = 0; w < wave; ++w) off += wsum[k][w];
+    if (v) {
+      la[off] = i0 + tid + 256 * k;
+      lb[off] = jv[k];
+    }
+    base += wsum[k][0] + wsum[k][1] + wsum[k][2] + wsum[k][3];
+  }
+  if (blk == nblk - 1 && tid == 0) p.total[f] = base;
 }
 
 // Rows staged per thread and chunk: 4 (256-row chunks) for 1- and 2-byte types,
@@ -110,6 +170,8 @@ __global__ __launch_bounds__(256) void wgrad_plan_kernel(WgradArgs p) {
 template <typename T> struct WgRows { static constexpr int value = 4; };
 template <> struct WgRows<float> { static constexpr int value = 2; };
 
+// Partial products over one piece of offset f's pair list: rows a[list_a] and
+// b[list_b], 64 x 64 output tile, FP32 partial per piece.
 template <typename T>
 __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs p) {
   using MM = Mma<T>;
@@ -127,8 +189,8 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs p) {
 
   const int f = blockIdx.y;
   if (!((p.mask_filter >> f) & 1u)) return;
-  const int count = p.gcount[f];
-  const int nb = wgrad_pieces(count, p.max_blocks, p.min_groups);
+  const int count = wgrad_total(p, f);
+  const int nb = wgrad_pieces(count, p.max_blocks, p.min_rows);
   const int piece = blockIdx.z;
   if (piece >= nb) return;
   const int tiles_n = (p.N + TS - 1) / TS;
@@ -139,11 +201,9 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs p) {
   const int lr = lane & 15, lg = lane >> 4;
   const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
   const int srow = tid & 63, sq = tid >> 6;
-  const int gpb = (count + nb - 1) / nb;
-  const int g0 = piece * gpb;
-  const int g1 = min(g0 + gpb, count);
-  const int nrows_blk = (g1 - g0) * 16;
-  const int* glist = p.glist + (int64_t)f * p.ngroups + g0;
+  const int per = (count + nb - 1) / nb;
+  const int q0 = piece * per;
+  const int nrows_blk = min(per, count - q0);
   const int nchunks = (nrows_blk + CROWS - 1) / CROWS;
 
   f32x4 acc[2][2];
@@ -152,86 +212,41 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs p) {
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-  // Every index and data load is an unconditional buffer load: invalid lanes
-  // use an out-of-range offset and read zeros, so no load sits behind a branch
-  // and all of a chunk's loads are in flight together.
+  // All loads are unconditional buffer loads (out-of-range offsets read 0).
   const __amdgpu_buffer_rsrc_t rA = make_rsrc(p.a, p.a_bytes);
   const __amdgpu_buffer_rsrc_t rB = make_rsrc(p.b, p.b_bytes);
-  const __amdgpu_buffer_rsrc_t rG = make_rsrc(glist, (uint32_t)(g1 - g0) * 4u);
-  const __amdgpu_buffer_rsrc_t rAr = make_rsrc(p.a_rows, (uint32_t)p.nrows * 4u);
-  const __amdgpu_buffer_rsrc_t rBr =
-      p.pairs ? make_rsrc(p.pairs + (int64_t)f * p.pair_ld, (uint32_t)p.nrows * 4u)
-              : make_rsrc(p.b_rows, (uint32_t)p.nrows * 4u);
-  const bool by_pairs = p.pairs != nullptr;
+  const int64_t lofs = (int64_t)f * p.list_ld + q0;
+  const __amdgpu_buffer_rsrc_t rLa = make_rsrc(p.list_a + lofs, (uint32_t)nrows_blk * 4u);
+  const __amdgpu_buffer_rsrc_t rLb = make_rsrc(p.list_b + lofs, (uint32_t)nrows_blk * 4u);
 
-  // Four-stage index pipeline per staged row: in the iteration of chunk j the
-  // thread loads the data rows of chunk j, the b-row index of chunk j+1, the
-  // a-row (argsort) index of chunk j+2 and the group id of chunk j+3, so each
-  // chunk waits for one round trip.  A value loaded in one iteration is first
-  // read in the next one.
-  auto posof = [&](int j, int st) { return j * CROWS + st * R + srow; };
-  auto ld_group = [&](int j, int st) -> int {
-    const int pos = posof(j, st);
-    return (int)__builtin_amdgcn_raw_buffer_load_b32(rG, pos < nrows_blk ? (uint32_t)(pos >> 4) * 4u : kBufOOB, 0, 0);
+  // Two-stage pipeline: the list entries of chunk j+1 load with the rows of chunk j.
+  auto ld_idx = [&](int j, int (&ra)[RPT], int (&rb)[RPT]) {
+#pragma unroll
+    for (int st = 0; st < RPT; ++st) {
+      const int pos = j * CROWS + st * R + srow;
+      const uint32_t off = pos < nrows_blk ? (uint32_t)pos * 4u : kBufOOB;
+      ra[st] = (int)__builtin_amdgcn_raw_buffer_load_b32(rLa, off, 0, 0);
+      rb[st] = (int)__builtin_amdgcn_raw_buffer_load_b32(rLb, off, 0, 0);
+    }
   };
-  int graw2[RPT], araw1[RPT], r1[RPT], brw0[RPT], aeff0[RPT];
-  bool v1[RPT];
-  {
-    int g0r[RPT], g1r[RPT];
-#pragma unroll
-    for (int st = 0; st < RPT; ++st) {
-      g0r[st] = ld_group(0, st);
-      g1r[st] = ld_group(1, st);
-      graw2[st] = ld_group(2, st);
-    }
-    int r0[RPT], araw0[RPT];
-    bool v0[RPT];
-#pragma unroll
-    for (int st = 0; st < RPT; ++st) {
-      const int p0 = posof(0, st), p1 = posof(1, st);
-      r0[st] = g0r[st] * 16 + (p0 & 15);
-      r1[st] = g1r[st] * 16 + (p1 & 15);
-      v0[st] = p0 < nrows_blk && r0[st] < p.nrows;
-      v1[st] = p1 < nrows_blk && r1[st] < p.nrows;
-      araw0[st] = (int)__builtin_amdgcn_raw_buffer_load_b32(rAr, v0[st] ? (uint32_t)r0[st] * 4u : kBufOOB, 0, 0);
-      araw1[st] = (int)__builtin_amdgcn_raw_buffer_load_b32(rAr, v1[st] ? (uint32_t)r1[st] * 4u : kBufOOB, 0, 0);
-    }
-#pragma unroll
-    for (int st = 0; st < RPT; ++st) {
-      aeff0[st] = v0[st] ? araw0[st] : -1;
-      const uint32_t off = by_pairs ? (uint32_t)aeff0[st] * 4u : (uint32_t)r0[st] * 4u;
-      brw0[st] = (int)__builtin_amdgcn_raw_buffer_load_b32(rBr, aeff0[st] >= 0 ? off : kBufOOB, 0, 0);
-    }
-  }
-
+  int ra0[RPT], rb0[RPT];
+  ld_idx(0, ra0, rb0);
   for (int j = 0; j < nchunks; ++j) {
     u32x4 va[RPT][CPT], vb[RPT][CPT];
 #pragma unroll
     for (int st = 0; st < RPT; ++st) {
-      const int rb = aeff0[st] >= 0 ? brw0[st] : -1;
-      const bool ok = rb >= 0;
+      const bool ok = j * CROWS + st * R + srow < nrows_blk;
 #pragma unroll
       for (int i = 0; i < CPT; ++i) {
         const int ch = (sq + 4 * i) * E;
         const bool oka = ok && ch < TS && m0 + ch < p.M;
         const bool okb = ok && ch < TS && nn0 + ch < p.N;
-        va[st][i] = bld16(rA, oka ? (uint32_t)(((int64_t)aeff0[st] * p.lda + m0 + ch) * (int)sizeof(T)) : kBufOOB);
-        vb[st][i] = bld16(rB, okb ? (uint32_t)(((int64_t)rb * p.ldb + nn0 + ch) * (int)sizeof(T)) : kBufOOB);
+        va[st][i] = bld16(rA, oka ? (uint32_t)(((int64_t)ra0[st] * p.lda + m0 + ch) * (int)sizeof(T)) : kBufOOB);
+        vb[st][i] = bld16(rB, okb ? (uint32_t)(((int64_t)rb0[st] * p.ldb + nn0 + ch) * (int)sizeof(T)) : kBufOOB);
       }
     }
-    int brw1[RPT], aeff1[RPT], araw2[RPT], r2[RPT], graw3[RPT];
-    bool v2[RPT];
-#pragma unroll
-    for (int st = 0; st < RPT; ++st) {
-      aeff1[st] = v1[st] ? araw1[st] : -1;
-      const uint32_t off = by_pairs ? (uint32_t)aeff1[st] * 4u : (uint32_t)r1[st] * 4u;
-      brw1[st] = (int)__builtin_amdgcn_raw_buffer_load_b32(rBr, aeff1[st] >= 0 ? off : kBufOOB, 0, 0);
-      const int p2 = posof(j + 2, st);
-      r2[st] = graw2[st] * 16 + (p2 & 15);
-      v2[st] = p2 < nrows_blk && r2[st] < p.nrows;
-      araw2[st] = (int)__builtin_amdgcn_raw_buffer_load_b32(rAr, v2[st] ? (uint32_t)r2[st] * 4u : kBufOOB, 0, 0);
-      graw3[st] = ld_group(j + 3, st);
-    }
+    int ra1[RPT], rb1[RPT];
+    ld_idx(j + 1, ra1, rb1);
 #pragma unroll
     for (int st = 0; st < RPT; ++st) {
       __syncthreads();  // previous stage's MFMAs are done with the tiles
@@ -261,12 +276,8 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs p) {
     }
 #pragma unroll
     for (int st = 0; st < RPT; ++st) {
-      aeff0[st] = aeff1[st];
-      brw0[st] = brw1[st];
-      araw1[st] = araw2[st];
-      r1[st] = r2[st];
-      v1[st] = v2[st];
-      graw2[st] = graw3[st];
+      ra0[st] = ra1[st];
+      rb0[st] = rb1[st];
     }
   }
 
@@ -284,51 +295,84 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs p) {
 }
 
 // ============================================================ split-K tree reduce
-template <typename OutT>
+// A block reduces 16 * V consecutive output elements.  Thread (q, e) loads V
+// consecutive elements of pieces b = q, q + 16, ... (one 16 * V-wide vector
+// load per piece, all in flight at once) and sums them; the 16 piece lanes of
+// each element are then added in a fixed pairwise tree: the order depends
+// only on the piece count, never on scheduling.  V = 4 when the element count
+// of an offset is a multiple of 4 (vector loads stay inside one offset).
+template <typename OutT, int V>
 __global__ __launch_bounds__(256) void splitk_reduce_kernel(ReduceArgs p) {
-  constexpr int L1 = 32;                     // first tree level
-  constexpr int L2 = kWgradMaxBlocks / L1;   // second tree level
+  constexpr int PQ = 16;                     // piece lanes per element
+  constexpr int PER = kWgradMaxBlocks / PQ;  // pieces per thread, at most
+  __shared__ float red[PQ][16 * V + 1];
   const int64_t mn = (int64_t)p.M * p.N;
-  const int64_t total = (int64_t)p.kv * mn;
-  OutT* out = reinterpret_cast<OutT*>(p.out);
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    const int f = (int)(e / mn);
-    if (!((p.mask_filter >> f) & 1u)) continue;
-    const int64_t rem = e - (int64_t)f * mn;
-    const int mm = (int)(rem / p.N);
-    const int nn = (int)(rem - (int64_t)mm * p.N);
-    const int nb = wgrad_pieces(p.gcount[f], p.max_blocks, p.min_groups);
-    const float* src = p.partial + (int64_t)f * p.max_blocks * mn + rem;
-    // fixed two-level pairwise tree over the pieces, independent of scheduling
-    float s2[L2];
+  const int tid = threadIdx.x;
+  const int el = tid & 15, q = tid >> 4;
+  const int64_t e = ((int64_t)blockIdx.x * 16 + el) * V;  // first of this thread's V elements
+  const bool live = e < (int64_t)p.kv * mn;
+  const int fe = live ? (int)(e / mn) : 0;
+  const int64_t rem = e - (int64_t)fe * mn;
+  const bool on = live && ((p.mask_filter >> fe) & 1u);
+  const int nb = on ? wgrad_pieces(p.fixed_total >= 0 ? p.fixed_total : p.total[fe], p.max_blocks, p.min_rows) : 0;
+  const __amdgpu_buffer_rsrc_t rp = make_rsrc(p.partial + (int64_t)fe * p.max_blocks * mn,
+                                              (uint32_t)((int64_t)nb * mn * 4));
+  float acc[V];
+  if constexpr (V == 4) {
+    f32x4 v[PER];
 #pragma unroll
-    for (int c = 0; c < L2; ++c) {
-      float v[L1];
-#pragma unroll
-      for (int s = 0; s < L1; ++s) {
-        const int b = c * L1 + s;
-        v[s] = b < nb ? src[(int64_t)b * mn] : 0.f;
-      }
-#pragma unroll
-      for (int w = L1 / 2; w >= 1; w >>= 1)
-#pragma unroll
-        for (int i = 0; i < w; ++i) v[i] = v[2 * i] + v[2 * i + 1];
-      s2[c] = v[0];
+    for (int k = 0; k < PER; ++k) {
+      const int b = q + PQ * k;
+      const u32x4 x = bld16(rp, b < nb ? (uint32_t)(((int64_t)b * mn + rem) * 4) : kBufOOB);
+      v[k] = f32x4{__uint_as_float(x.x), __uint_as_float(x.y), __uint_as_float(x.z), __uint_as_float(x.w)};
     }
 #pragma unroll
-    for (int w = L2 / 2; w >= 1; w >>= 1)
+    for (int w = PER / 2; w >= 1; w >>= 1)
 #pragma unroll
-      for (int i = 0; i < w; ++i) s2[i] = s2[2 * i] + s2[2 * i + 1];
-    float r = p.alpha * s2[0];
-    const int64_t oi = (int64_t)mm * p.o_ms + (int64_t)f * p.o_fs + nn;
-    if (p.beta != 0.f) {
-      const OutT* sp = reinterpret_cast<const OutT*>(p.src);
-      r += p.beta * to_f32(p.src_is_bias ? sp[nn] : sp[(int64_t)mm * p.s_ms + (int64_t)f * p.s_fs + nn]);
+      for (int i = 0; i < w; ++i) v[i] = v[2 * i] + v[2 * i + 1];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[c] = v[0][c];
+  } else {
+    float v[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int b = q + PQ * k;
+      v[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+          rp, b < nb ? (uint32_t)(((int64_t)b * mn + rem) * 4) : kBufOOB, 0, 0));
     }
-    r = apply_act(r, p.act_type, p.act_alpha);
-    out[oi] = from_f32<OutT>(r);
+#pragma unroll
+    for (int w = PER / 2; w >= 1; w >>= 1)
+#pragma unroll
+      for (int i = 0; i < w; ++i) v[i] = v[2 * i] + v[2 * i + 1];
+    acc[0] = v[0];
   }
+#pragma unroll
+  for (int c = 0; c < V; ++c) red[q][el * V + c] = acc[c];
+  __syncthreads();
+  // 16 * V elements, 256 threads: thread tid finishes element tid (if any)
+  if (tid >= 16 * V) return;
+  const int64_t eo = (int64_t)blockIdx.x * 16 * V + tid;
+  if (eo >= (int64_t)p.kv * mn) return;
+  const int fo = (int)(eo / mn);
+  if (!((p.mask_filter >> fo) & 1u)) return;
+  float t[PQ];
+#pragma unroll
+  for (int i = 0; i < PQ; ++i) t[i] = red[i][tid];
+#pragma unroll
+  for (int w = PQ / 2; w >= 1; w >>= 1)
+#pragma unroll
+    for (int i = 0; i < w; ++i) t[i] = t[2 * i] + t[2 * i + 1];
+  const int64_t ro = eo - (int64_t)fo * mn;
+  const int mm = (int)(ro / p.N);
+  const int nn = (int)(ro - (int64_t)mm * p.N);
+  float r = p.alpha * t[0];
+  const int64_t oi = (int64_t)mm * p.o_ms + (int64_t)fo * p.o_fs + nn;
+  if (p.beta != 0.f) {
+    const OutT* sp = reinterpret_cast<const OutT*>(p.src);
+    r += p.beta * to_f32(p.src_is_bias ? sp[nn] : sp[(int64_t)mm * p.s_ms + (int64_t)fo * p.s_fs + nn]);
+  }
+  r = apply_act(r, p.act_type, p.act_alpha);
+  reinterpret_cast<OutT*>(p.out)[oi] = from_f32<OutT>(r);
 }
 
 // ============================================================ helpers
@@ -388,10 +432,20 @@ hipError_t launch_rowgather(int in_dtype, int out_dtype, int block_n, const RowG
 }
 
 hipError_t launch_wgrad(int in_dtype, const WgradArgs& a, hipStream_t s) {
-  if (a.max_blocks < 1 || a.max_blocks > kWgradMaxBlocks || a.min_groups < 1) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(wgrad_plan_kernel, dim3(a.kv), dim3(256), 0, s, a);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  if (a.max_blocks < 1 || a.max_blocks > kWgradMaxBlocks || a.min_rows < 1) return hipErrorInvalidValue;
+  hipError_t e;
+  if (a.fixed_total < 0) {
+    // conv form: effective row masks, then the per-offset pair lists
+    if (a.nrows > 0) {
+      hipLaunchKernelGGL(wgrad_rowmask_kernel, dim3((a.nrows + 255) / 256), dim3(256), 0, s, a);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    const int nblk = std::max(1, (a.nrows + kWgradListRows - 1) / kWgradListRows);
+    hipLaunchKernelGGL(wgrad_count_kernel, dim3(nblk, a.kv), dim3(256), 0, s, a);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(wgrad_list_kernel, dim3(nblk, a.kv), dim3(256), 0, s, a);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
   const int tiles = ((a.M + kWgradTile - 1) / kWgradTile) * ((a.N + kWgradTile - 1) / kWgradTile);
   dim3 grid(tiles, a.kv, a.max_blocks);
   dim3 block(256, 1, 1);
@@ -408,11 +462,14 @@ hipError_t launch_wgrad(int in_dtype, const WgradArgs& a, hipStream_t s) {
 
 template <typename OutT>
 static hipError_t launch_reduce_t(const ReduceArgs& a, hipStream_t s) {
-  const int64_t total = (int64_t)a.kv * a.M * a.N;
-  int blocks = (int)std::min<int64_t>((total + 255) / 256, 4096);
-  if (blocks == 0) return hipSuccess;
+  const int64_t mn = (int64_t)a.M * a.N;
+  const int64_t total = (int64_t)a.kv * mn;
+  if (total == 0) return hipSuccess;
   if (a.max_blocks < 1 || a.max_blocks > kWgradMaxBlocks) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((splitk_reduce_kernel<OutT>), dim3(blocks), dim3(256), 0, s, a);
+  if (mn % 4 == 0)
+    hipLaunchKernelGGL((splitk_reduce_kernel<OutT, 4>), dim3((unsigned)((total + 63) / 64)), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((splitk_reduce_kernel<OutT, 1>), dim3((unsigned)((total + 15) / 16)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

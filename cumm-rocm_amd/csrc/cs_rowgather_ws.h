@@ -245,7 +245,13 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
   // of tile k+1 and the row data of tile k+2 load too, so a tile costs about
   // one round trip.  Tiles with more than BATCH offsets finish their extra
   // batches inline.
-  int t = blockIdx.x * kWsWaves + wave;
+  const int k0 = blockIdx.x * kWsWaves + wave;
+  auto tile_at = [&](int j) -> int {  // j: task ordinal of this wave (uniform)
+    const int k = k0 + j * G;
+    return k < ntiles ? k : ntiles;
+  };
+  int j = 0;
+  int t = tile_at(0);
   if (t >= ntiles) return;
   Meta Mc, Mn;
   load_meta(t, Mc);
@@ -253,12 +259,13 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
   uint32_t rem_c = prepare(tile_mask(t, Mc.m), live_c);
   int ia_c, ib_c;
   uint32_t bm_c = batch(rem_c, Mc.col, ia_c, ib_c);
-  load_meta(t + G, Mn);
+  int tn = tile_at(1);
+  load_meta(tn, Mn);
   while (true) {
     u32x4 af[BATCH][KC];
     gather(bm_c, ia_c, ib_c, af);
     // next tile: its mask is final once its row data has landed
-    const int tn = t + G;
+    const int tnn = tile_at(j + 2);
     bool live_n = false;
     uint32_t rem_n = 0u, bm_n = 0u;
     int ia_n = -1, ib_n = -1;
@@ -266,7 +273,7 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
     if (tn < ntiles) {
       rem_n = prepare(tile_mask(tn, Mn.m), live_n);
       bm_n = batch(rem_n, Mn.col, ia_n, ib_n);
-      load_meta(tn + G, Mnn);
+      load_meta(tnn, Mnn);
     }
     f32x4 acc[NBC];
 #pragma unroll
@@ -280,7 +287,9 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
     }
     if (live_c) epilogue(t, Mc.orow, acc);
     if (tn >= ntiles) break;
+    ++j;
     t = tn;
+    tn = tnn;
     Mc = Mn;
     Mn = Mnn;
     live_c = live_n;
