@@ -38,6 +38,7 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
   constexpr int BATCH = KC > 2 ? 4 : 8;  // offsets per gather round trip
   __shared__ __attribute__((aligned(16))) char wlds[kWsLdsBytes];
   __shared__ __attribute__((aligned(16))) char stage_lds[kWsWaves * kWsStageBytes];
+  __shared__ int tile_ctr;  // next entry of this block's tile pool
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -73,6 +74,7 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
       }
     }
   }
+  if (tid == 0) tile_ctr = 0;
   __syncthreads();
 
   const __amdgpu_buffer_rsrc_t ra = make_rsrc(p.a, p.a_bytes);
@@ -245,13 +247,18 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
   // of tile k+1 and the row data of tile k+2 load too, so a tile costs about
   // one round trip.  Tiles with more than BATCH offsets finish their extra
   // batches inline.
-  const int k0 = blockIdx.x * kWsWaves + wave;
-  auto tile_at = [&](int j) -> int {  // j: task ordinal of this wave (uniform)
-    const int k = k0 + j * G;
-    return k < ntiles ? k : ntiles;
+  // Block b's pool is tiles b, b + gridDim.x, b + 2 gridDim.x, ... (a uniform
+  // sample of the tile order, so pools weigh about the same); its waves take
+  // the pool's entries in turn through an LDS counter, so a wave that drew
+  // heavy tiles takes fewer.  Results do not depend on which wave runs a tile.
+  auto next_tile = [&]() -> int {
+    int v = 0;
+    if (lane == 0) v = atomicAdd(&tile_ctr, 1);
+    v = __builtin_amdgcn_readfirstlane(v);
+    const int64_t k = (int64_t)blockIdx.x + (int64_t)gridDim.x * v;
+    return k < ntiles ? (int)k : ntiles;
   };
-  int j = 0;
-  int t = tile_at(0);
+  int t = next_tile();
   if (t >= ntiles) return;
   Meta Mc, Mn;
   load_meta(t, Mc);
@@ -259,13 +266,13 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
   uint32_t rem_c = prepare(tile_mask(t, Mc.m), live_c);
   int ia_c, ib_c;
   uint32_t bm_c = batch(rem_c, Mc.col, ia_c, ib_c);
-  int tn = tile_at(1);
+  int tn = next_tile();
   load_meta(tn, Mn);
   while (true) {
     u32x4 af[BATCH][KC];
     gather(bm_c, ia_c, ib_c, af);
     // next tile: its mask is final once its row data has landed
-    const int tnn = tile_at(j + 2);
+    const int tnn = tn < ntiles ? next_tile() : ntiles;
     bool live_n = false;
     uint32_t rem_n = 0u, bm_n = 0u;
     int ia_n = -1, ib_n = -1;
@@ -287,7 +294,6 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
     }
     if (live_c) epilogue(t, Mc.orow, acc);
     if (tn >= ntiles) break;
-    ++j;
     t = tn;
     tn = tnn;
     Mc = Mn;
