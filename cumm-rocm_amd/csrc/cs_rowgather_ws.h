@@ -46,6 +46,28 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
   const int lr = lane & 15, lg = lane >> 4;
   const int n0 = blockIdx.y * NCOL;
 
+  const uint32_t kvmask = p.kv >= 32 ? 0xffffffffu : ((1u << p.kv) - 1u);
+  // per-task row data: row mask, pair column (argsort) and output row of the
+  // lane's row (lr); rows past m read as empty
+  struct Meta { uint32_t m; int col, orow; };
+  auto load_meta = [&](int t, Meta& M) {
+    const int r = t * kMaskTile + lr;
+    M.m = 0u;
+    M.col = 0;
+    M.orow = 0;
+    if (t < ntiles && r < p.m) {
+      M.m = p.mask ? p.mask[r] : kvmask;
+      M.col = p.pair_col ? p.pair_col[r] : r;
+      M.orow = p.out_row ? p.out_row[r] : r;
+    }
+  };
+
+  // Each wave's first two tiles are fixed (pool entries wave and
+  // kWsWaves + wave); their row data loads together with the weights.
+  const int t0 = (int)std::min<int64_t>((int64_t)blockIdx.x + (int64_t)gridDim.x * wave, ntiles);
+  const int t1 = (int)std::min<int64_t>((int64_t)blockIdx.x + (int64_t)gridDim.x * (kWsWaves + wave), ntiles);
+  Meta Mc, Mn;
+
   // ---- weights of all offsets for columns [n0, n0 + NCOL) -> LDS.  Each
   // thread issues its (at most WPT) loads before writing any of them, so the
   // preload costs one round trip instead of one per chunk.
@@ -63,6 +85,8 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
       const bool ok = i < total && n0 + n < p.nout;
       v[j] = ok ? ld16(B + (int64_t)f * p.b_fs + (int64_t)(n0 + n) * p.b_ns + c * E) : zero16();
     }
+    load_meta(t0, Mc);
+    load_meta(t1, Mn);
 #pragma unroll
     for (int j = 0; j < WPT; ++j) {
       const int i = tid + j * (64 * kWsWaves);
@@ -74,14 +98,12 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
       }
     }
   }
-  if (tid == 0) tile_ctr = 0;
+  if (tid == 0) tile_ctr = 2 * kWsWaves;
   __syncthreads();
 
   const __amdgpu_buffer_rsrc_t ra = make_rsrc(p.a, p.a_bytes);
-  const uint32_t kvmask = p.kv >= 32 ? 0xffffffffu : ((1u << p.kv) - 1u);
   OutT* Cout = reinterpret_cast<OutT*>(p.c);
   const OutT* D = reinterpret_cast<const OutT*>(p.d);
-  const int G = gridDim.x * kWsWaves;
   // coalesced row stores through a per-wave LDS tile (beta == 0, whole slice
   // inside nout, 16-byte aligned rows)
   constexpr int ROWB = NCOL * (int)sizeof(OutT);
@@ -91,20 +113,6 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
                          ((reinterpret_cast<uintptr_t>(Cout) + (uintptr_t)n0 * sizeof(OutT)) & 15u) == 0;
   char* stage = stage_lds + wave * kWsStageBytes;
 
-  // per-task row data: row mask, pair column (argsort) and output row of the
-  // lane's row (lr); rows past m read as empty
-  struct Meta { uint32_t m; int col, orow; };
-  auto load_meta = [&](int t, Meta& M) {
-    const int r = t * kMaskTile + lr;
-    M.m = 0u;
-    M.col = 0;
-    M.orow = 0;
-    if (t < ntiles && r < p.m) {
-      M.m = p.mask ? p.mask[r] : kvmask;
-      M.col = p.pair_col ? p.pair_col[r] : r;
-      M.orow = p.out_row ? p.out_row[r] : r;
-    }
-  };
   // tile mask (also stored to mask_output), filter, reverse
   auto tile_mask = [&](int t, uint32_t m) -> uint32_t {
     m |= __shfl_xor(m, 1);
@@ -242,11 +250,10 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
     }
   };
 
-  // Tiles t, t + G, t + 2G, ... of this wave, software-pipelined across
-  // tiles: while the gathered rows of tile k are in flight, the pair indices
-  // of tile k+1 and the row data of tile k+2 load too, so a tile costs about
-  // one round trip.  Tiles with more than BATCH offsets finish their extra
-  // batches inline.
+  // The wave's tiles are software-pipelined: while the gathered rows of tile
+  // k are in flight, the pair indices of tile k+1 and the row data of tile
+  // k+2 load too, so a tile costs about one round trip.  Tiles with more than
+  // BATCH offsets finish their extra batches inline.
   // Block b's pool is tiles b, b + gridDim.x, b + 2 gridDim.x, ... (a uniform
   // sample of the tile order, so pools weigh about the same); its waves take
   // the pool's entries in turn through an LDS counter, so a wave that drew
@@ -258,16 +265,13 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
     const int64_t k = (int64_t)blockIdx.x + (int64_t)gridDim.x * v;
     return k < ntiles ? (int)k : ntiles;
   };
-  int t = next_tile();
+  int t = t0;
   if (t >= ntiles) return;
-  Meta Mc, Mn;
-  load_meta(t, Mc);
   bool live_c;
   uint32_t rem_c = prepare(tile_mask(t, Mc.m), live_c);
   int ia_c, ib_c;
   uint32_t bm_c = batch(rem_c, Mc.col, ia_c, ib_c);
-  int tn = next_tile();
-  load_meta(tn, Mn);
+  int tn = t1;
   while (true) {
     u32x4 af[BATCH][KC];
     gather(bm_c, ia_c, ib_c, af);
