@@ -450,7 +450,8 @@ def run_serving(args):
         arr = np.cumsum(r.exponential(1.0 / rate, n))
         return [Frame(i, pool[j], feats[j]) for i, j in enumerate(idx)], arr
 
-    forms = {"bf16_continuous": dict(dt="bf16", max_batch=16, n_streams=2),
+    forms = {"bf16_continuous": dict(dt="bf16", max_batch=16, n_streams=1),
+             "bf16_continuous_2streams": dict(dt="bf16", max_batch=16, n_streams=2),
              "f32_per_call": dict(dt="f32", max_batch=1, n_streams=1)}
     res = {}
     for name, kw in forms.items():

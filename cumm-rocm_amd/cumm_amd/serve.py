@@ -62,7 +62,7 @@ class _Batch:
 
 class SparseConvServer:
     def __init__(self, kinds: Sequence[str], channels: Sequence[int], weights: Sequence[np.ndarray], dt: str = "bf16",
-                 hbm_budget: int = 8 << 30, max_inflight: int = 64, max_batch: int = 16, n_streams: int = 2,
+                 hbm_budget: int = 8 << 30, max_inflight: int = 64, max_batch: int = 16, n_streams: int = 1,
                  grid=GRID, keep_outputs: bool = True):
         assert len(kinds) + 1 == len(channels) == len(weights) + 1
         self.kinds, self.channels, self.dt = tuple(kinds), tuple(channels), dt
@@ -72,6 +72,10 @@ class SparseConvServer:
         self.reserved = 0
         self.max_inflight, self.max_batch = max_inflight, max_batch
         self.streams = [torch.cuda.Stream() for _ in range(n_streams)]
+        # pinned host staging per stream, grown on demand and reused (a fresh
+        # pinned allocation per batch costs milliseconds)
+        self._pin_c: List[Optional[torch.Tensor]] = [None] * n_streams
+        self._pin_x: List[Optional[torch.Tensor]] = [None] * n_streams
         self.busy: Dict[int, Optional[_Batch]] = {i: None for i in range(n_streams)}
         self.grid = grid
         self.keep_outputs = keep_outputs
@@ -135,9 +139,12 @@ class SparseConvServer:
             f.start = now
             f.batch_size = len(frames)
         feats = np.concatenate([f.feats for f in frames])
+        pc_, px_ = self._staging(si, coords.shape[0], feats.shape[1])
+        pc_[:coords.shape[0]].numpy()[...] = coords
+        px_[:feats.shape[0]].numpy()[...] = feats
         with torch.cuda.stream(s):
-            c_dev = torch.from_numpy(coords).pin_memory().to("cuda", non_blocking=True)
-            x_dev = torch.from_numpy(feats).pin_memory().to("cuda", non_blocking=True).to(B.TORCH_DT[self.dt])
+            c_dev = pc_[:coords.shape[0]].to("cuda", non_blocking=True)
+            x_dev = px_[:feats.shape[0]].to("cuda", non_blocking=True).to(B.TORCH_DT[self.dt])
             tabs = pairs.stack_tables(self.kinds, c_dev, self.grid, stream=s.cuda_stream)
             st = B.ConvStack(tabs, self.channels, self.dt, self.weights, s.cuda_stream, train=False)
             st.bind(x_dev)
@@ -146,6 +153,16 @@ class SparseConvServer:
             ev.record(s)
         self.busy[si] = _Batch(frames, s, ev, [c_dev, x_dev, tabs, st], sum(self.frame_bytes(f.coords.shape[0])
                                                                              for f in frames))
+
+    def _staging(self, si: int, n: int, c: int):
+        """Pinned buffers of stream si holding at least n rows.  A buffer is
+        refilled only once the stream's previous batch (whose upload read it)
+        has completed, which _launch guarantees by taking idle streams only."""
+        if self._pin_c[si] is None or self._pin_c[si].shape[0] < n or self._pin_x[si].shape[1] != c:
+            cap = max(n, 1 << 16)
+            self._pin_c[si] = torch.empty((cap, 4), dtype=torch.int32, pin_memory=True)
+            self._pin_x[si] = torch.empty((cap, c), dtype=torch.float32, pin_memory=True)
+        return self._pin_c[si], self._pin_x[si]
 
     def _poll(self) -> None:
         for si, b in self.busy.items():
