@@ -620,13 +620,15 @@ void run_conv(const cs_conv_params& p) {
       a.b_ns = plan.kv * plan.C;
       a.b_bytes = (uint32_t)span_bytes(p.weight);
     } else {
-      // B_f[n=c][k] needs W transposed to [kv][C][K]
-      CS_HIP(cs::launch_transpose(p.weight.data, ws.ptr, plan.K, plan.kv, plan.C, plan.kv * plan.C,
-                                  dtype_bytes(p.weight.dtype), s));
-      a.b = ws.ptr;
-      a.b_fs = plan.C * plan.K;
-      a.b_ns = plan.K;
-      a.b_bytes = (uint32_t)(plan.kv * plan.C * plan.K * dtype_bytes(p.weight.dtype));
+      // B_f[n=c][k] = W[k][f][c]: read k-major from the weights themselves; a
+      // kernel that needs it n-major transposes into the workspace
+      a.b = p.weight.data;
+      a.b_kmajor = 1;
+      a.b_ks = plan.kv * plan.C;
+      a.b_fs = plan.C;
+      a.b_ns = 1;
+      a.b_scratch = ws.ptr;
+      a.b_bytes = (uint32_t)span_bytes(p.weight);
     }
     a.c = dst.data;
     a.ldc = row_stride(dst);

@@ -22,6 +22,9 @@ enum DT : int {
 struct RowGatherArgs {
   const void* a; int64_t lda;
   const void* b; int64_t b_fs; int64_t b_ns;
+  // b_kmajor: B_f[n][k] = b[k*b_ks + f*b_fs + n] instead (dgrad's KRSC weights);
+  // kernels that need the n-major form build it in b_scratch ([kv][nout][kred])
+  int32_t b_kmajor; int64_t b_ks; void* b_scratch;
   uint32_t a_bytes, b_bytes;  // extents of a and b (< 2^31): buffer-load range
   void* c; int64_t ldc;
   const void* d; int64_t ldd; int32_t d_is_bias;

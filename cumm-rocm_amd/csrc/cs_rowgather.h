@@ -228,6 +228,19 @@ static hipError_t launch_rg(const RowGatherArgs& a, int block_n, hipStream_t s) 
   bool used = false;
   const hipError_t e = launch_ws<T, OutT>(a, block_n, s, &used);
   if (used || e != hipSuccess) return e;
+  if (a.b_kmajor) {
+    // the block kernel reads B n-major: transpose [kred][kv][nout] -> [kv][nout][kred]
+    if (a.b_scratch == nullptr) return hipErrorInvalidValue;
+    const hipError_t et = launch_transpose(a.b, a.b_scratch, a.kred, a.kv, a.nout, a.b_ks, (int)sizeof(T), s);
+    if (et != hipSuccess) return et;
+    RowGatherArgs a2 = a;
+    a2.b = a.b_scratch;
+    a2.b_kmajor = 0;
+    a2.b_fs = (int64_t)a.nout * a.kred;
+    a2.b_ns = a.kred;
+    a2.b_bytes = (uint32_t)((int64_t)a.kv * a.nout * a.kred * (int64_t)sizeof(T));
+    return launch_rg<T, OutT>(a2, block_n, s);
+  }
   dim3 grid((a.m + kRgRows - 1) / kRgRows, (a.nout + 63) / 64, 1);
   dim3 block(256, 1, 1);
   if (grid.x == 0 || grid.y == 0) return hipSuccess;

@@ -71,7 +71,52 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
   // ---- weights of all offsets for columns [n0, n0 + NCOL) -> LDS.  Each
   // thread issues its (at most WPT) loads before writing any of them, so the
   // preload costs one round trip instead of one per chunk.
-  {
+  if (p.b_kmajor) {
+    // B element (f, n, k) at b[k * b_ks + f * b_fs + n] (dgrad reads the KRSC
+    // weights directly): a unit of E reduction rows x E columns loads as E
+    // 16-byte rows and is transposed in registers into E chunks along k.
+    constexpr int S = (int)sizeof(T);
+    constexpr int NU = NCOL / E, KU = C / E;
+    constexpr int UPT = (32 * NU * KU + 64 * kWsWaves - 1) / (64 * kWsWaves);  // kv <= 32
+    const T* B = reinterpret_cast<const T*>(p.b);
+    const int units = p.kv * NU * KU;
+    u32x4 v[UPT][E];
+#pragma unroll
+    for (int j = 0; j < UPT; ++j) {
+      const int u = tid + j * (64 * kWsWaves);
+      const int f = u / (NU * KU), nu = (u / KU) % NU, ku = u % KU;
+      const bool ok = u < units && n0 + nu * E < p.nout;
+#pragma unroll
+      for (int i = 0; i < E; ++i)
+        v[j][i] = ok ? ld16(B + (int64_t)(ku * E + i) * p.b_ks + (int64_t)f * p.b_fs + n0 + nu * E) : zero16();
+    }
+    load_meta(t0, Mc);
+    load_meta(t1, Mn);
+#pragma unroll
+    for (int j = 0; j < UPT; ++j) {
+      const int u = tid + j * (64 * kWsWaves);
+      if (u < units) {
+        const int f = u / (NU * KU), nu = (u / KU) % NU, ku = u % KU;
+#pragma unroll
+        for (int jn = 0; jn < E; ++jn) {
+          u32x4 o = zero16();
+#pragma unroll
+          for (int i = 0; i < E; ++i) {
+            const uint32_t w = v[j][i][(jn * S) >> 2];
+            uint32_t e;
+            if constexpr (S == 4) {
+              e = w;
+            } else {
+              e = (w >> (((jn * S) & 3) * 8)) & ((1u << (8 * S)) - 1u);
+            }
+            o[(i * S) >> 2] |= e << (((i * S) & 3) * 8);
+          }
+          const int n = nu * E + jn;
+          *reinterpret_cast<u32x4*>(wlds + ((f * NCOL + n) * CHUNKS + (ku ^ (n & SWZ))) * 16) = o;
+        }
+      }
+    }
+  } else {
     const T* B = reinterpret_cast<const T*>(p.b);
     const int total = p.kv * NCOL * CHUNKS;
     constexpr int WPT = (kWsLdsBytes / 16 + 64 * kWsWaves - 1) / (64 * kWsWaves);

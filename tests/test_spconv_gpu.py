@@ -245,3 +245,21 @@ def test_wgrad_fp8_deterministic():
     assert np.array_equal(res[0].view(np.int32), res[1].view(np.int32))
     exp = ref.conv_wgrad(xq64, dq64, prob.pair_fwd, mo, prob.argsort, 16) * (s_do * s_x)
     assert _rel(res[0].astype(np.float64), exp) <= 1e-4
+
+
+@pytest.mark.parametrize("C,K", [(32, 16), (16, 48)])
+def test_dgrad_block_kernel_path(C, K):
+    """dgrad whose reduction (K) is not a multiple of the f16 MFMA chunk runs
+    the block kernel, which reads the weights through a transposed copy."""
+    c = subm_case(1500, C, K, seed=C * K)
+    prob = c.prob
+    print(f"[dgrad-block] C={C} K={K}", flush=True)
+    desp = desp_for(1, "f16", C, K)
+    din = torch.zeros((prob.n, C), dtype=torch.float16, device="cuda")
+    p = conv_params(1, desp, din, to_dev(c.W, "f16"), to_dev(c.dO, "f16"), prob.pair_bwd, prob.mask_sorted,
+                    prob.argsort, reverse_mask=True)
+    ConvMainUnitTest.implicit_gemm2(p)
+    torch.cuda.synchronize()
+    exp = ref.conv_dgrad(quantize(c.dO, "f16"), quantize(c.W, "f16"), prob.pair_bwd, prob.mask_sorted,
+                         prob.argsort, 16, reverse_mask=True)
+    assert _rel(din.double().cpu().numpy(), quantize(exp, "f16")) <= TOL["f16"]
