@@ -350,8 +350,9 @@ const std::vector<cs_conv_algo_desp>& conv_catalogue() {
   static const std::vector<cs_conv_algo_desp> cat = [] {
     std::vector<cs_conv_algo_desp> v;
     const ConvCombo fwd_combos[] = {{0, 0, 0}, {7, 7, 7}, {12, 12, 12}, {14, 14, 7}, {14, 14, 12}};
-    // tile_shape[1] = column slice of the weight-stationary kernel (the tuner's knob)
-    for (int bn : {64, 32, 16}) {
+    // tile_shape[1]: column slice of the weight-stationary kernel (16 / 32 / 64;
+    // 64 switches to streamed weights past two slices), 128 = streamed weights
+    for (int bn : {64, 32, 16, 128}) {
       for (auto c : fwd_combos) {
         // forward: a = input, b = weight, c = output; trans (F, T, F)
         v.push_back(conv_desp(0, rowgather_desp(c.in, c.w, c.out, cs::kMaskTile, bn, 0, 1, 0)));

@@ -18,8 +18,10 @@
 // through the epilogue.
 // Instantiated per input dtype in rg_<dtype>.hip.
 #pragma once
+#include <climits>
 #include "cs_device.h"
 #include "cs_kernels.h"
+#include "cs_rowgather_stream.h"
 #include "cs_rowgather_ws.h"
 
 namespace cs {
@@ -220,16 +222,30 @@ __global__ __launch_bounds__(256) void rowgather_kernel(RowGatherArgs p) {
   }
 }
 
+// Kernel choice.  block_n (the algo's tile_shape[1]) of 16 / 32 / 64 caps the
+// weight-stationary kernel's column slice; with 64 that kernel runs when it
+// needs at most two column slices and the streamed-weights kernel when it
+// would need more (every slice gathers the input rows again); 128 asks for
+// the streamed-weights kernel.  The block kernel covers the rest.
+template <typename T>
+static int ws_slices(const RowGatherArgs& a) {
+  int ncol = std::min(ws_ncol<T>(a.kv, a.kred), 64);
+  if (ncol == 48) ncol = 32;
+  return ncol > 0 ? (a.nout + ncol - 1) / ncol : INT_MAX;
+}
+
 template <typename T, typename OutT>
-// block_n (the algo's tile_shape[1]) caps the weight-stationary kernel's
-// column slice (16, 32 or 64); the block kernel always uses 64 columns.
 static hipError_t launch_rg(const RowGatherArgs& a, int block_n, hipStream_t s) {
-  if (block_n != 16 && block_n != 32 && block_n != 64) return hipErrorInvalidValue;
+  if (block_n != 16 && block_n != 32 && block_n != 64 && block_n != 128) return hipErrorInvalidValue;
   bool used = false;
-  const hipError_t e = launch_ws<T, OutT>(a, block_n, s, &used);
-  if (used || e != hipSuccess) return e;
+  hipError_t e;
+  const bool prefer_stream = block_n == 128 || (block_n == 64 && ws_slices<T>(a) > 2);
+  if (!prefer_stream) {
+    e = launch_ws<T, OutT>(a, block_n, s, &used);
+    if (used || e != hipSuccess) return e;
+  }
   if (a.b_kmajor) {
-    // the block kernel reads B n-major: transpose [kred][kv][nout] -> [kv][nout][kred]
+    // the other kernels read B n-major: transpose [kred][kv][nout] -> [kv][nout][kred]
     if (a.b_scratch == nullptr) return hipErrorInvalidValue;
     const hipError_t et = launch_transpose(a.b, a.b_scratch, a.kred, a.kv, a.nout, a.b_ks, (int)sizeof(T), s);
     if (et != hipSuccess) return et;
@@ -240,6 +256,12 @@ static hipError_t launch_rg(const RowGatherArgs& a, int block_n, hipStream_t s) 
     a2.b_ns = a.kred;
     a2.b_bytes = (uint32_t)((int64_t)a.kv * a.nout * a.kred * (int64_t)sizeof(T));
     return launch_rg<T, OutT>(a2, block_n, s);
+  }
+  e = launch_stream<T, OutT>(a, s, &used);
+  if (used || e != hipSuccess) return e;
+  if (prefer_stream) {
+    e = launch_ws<T, OutT>(a, 64, s, &used);
+    if (used || e != hipSuccess) return e;
   }
   dim3 grid((a.m + kRgRows - 1) / kRgRows, (a.nout + 63) / 64, 1);
   dim3 block(256, 1, 1);

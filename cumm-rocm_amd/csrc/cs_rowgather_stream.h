@@ -1,0 +1,238 @@
+// Streamed-weights row-gather kernel (sparse conv forward / dgrad with wide
+// layers, e.g. 128 -> 128 channels).
+//
+// When the weights of all kernel offsets for every output column do not fit
+// the LDS, the weight-stationary kernel has to split the columns into slices
+// and every slice gathers all input rows again.  Here a block owns 128 sorted
+// rows (eight 16-row mask tiles, one per wave) and all output columns, and
+// walks the union of its tiles' offset masks in ascending order.  For each
+// offset the block stages that offset's weights [nout][kred] in LDS (double
+// buffered) and each wave gathers its 16 rows straight into MFMA operand
+// registers; a wave whose own tile lacks the offset skips the MFMAs.  The
+// weights of the next offset and the rows of the next offset load while the
+// current offset's MFMAs run, and the pair indices two offsets ahead.
+// A row's sum runs over offsets in ascending order and a missing neighbour
+// adds an exact zero, so results are bit-identical to the other row-gather
+// kernels.
+#pragma once
+#include "cs_device.h"
+#include "cs_kernels.h"
+
+namespace cs {
+
+constexpr int kStWaves = 8;                 // waves per block = mask tiles per block
+constexpr int kStLdsBytes = 2 * 32768;      // two weight stages of up to 32 KiB
+
+// KC: reduction chunks (kred / chunk); NB: 16-column blocks (nout <= 16 NB)
+template <typename T, typename OutT, int KC, int NB>
+__global__ __launch_bounds__(64 * kStWaves) void rowgather_stream_kernel(RowGatherArgs p, int ntiles) {
+  using MM = Mma<T>;
+  constexpr int E = MM::kElems;
+  constexpr int CH = MM::kChunk;
+  constexpr int C = KC * CH;              // reduction length (== p.kred)
+  constexpr int NCOL = NB * 16;
+  constexpr int CHUNKS = C / E;           // 16-byte chunks per weight row
+  constexpr int SWZ = CHUNKS >= 8 ? 7 : CHUNKS - 1;
+  constexpr int STAGE = NCOL * CHUNKS * 16;  // bytes of one offset's weights
+  static_assert(2 * STAGE <= kStLdsBytes, "weight stage too large");
+  constexpr int NT = 64 * kStWaves;
+  constexpr int WCH = (NCOL * CHUNKS + NT - 1) / NT;  // 16-byte weight chunks per thread
+  __shared__ __attribute__((aligned(16))) char wlds[kStLdsBytes];
+  __shared__ uint32_t wmask[kStWaves];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int lr = lane & 15, lg = lane >> 4;
+  const uint32_t kvmask = p.kv >= 32 ? 0xffffffffu : ((1u << p.kv) - 1u);
+  const int t = blockIdx.x * kStWaves + wave;
+  const int row0 = t * kMaskTile;
+  const int r = row0 + lr;
+  const bool rvalid = t < ntiles && r < p.m;
+
+  // ---- this wave's tile: masks, pair column, output row of row lr
+  uint32_t m = 0u;
+  int col = 0, orow = 0;
+  if (rvalid) {
+    m = p.mask ? p.mask[r] : kvmask;
+    col = p.pair_col ? p.pair_col[r] : r;
+    orow = p.out_row ? p.out_row[r] : r;
+  }
+  m |= __shfl_xor(m, 1);
+  m |= __shfl_xor(m, 2);
+  m |= __shfl_xor(m, 4);
+  m |= __shfl_xor(m, 8);
+  uint32_t km = __builtin_amdgcn_readfirstlane(m) & p.mask_filter & kvmask;
+  if (t < ntiles && p.mask_out != nullptr && lane == 0) p.mask_out[t] = km;
+  const bool live = t < ntiles && !(p.skip_empty && km == 0u);
+  if (!live) km = 0u;
+  if (p.reverse_mask && km) km = __builtin_bitreverse32(km) >> (32 - p.kv);
+  if (lane == 0) wmask[wave] = km;
+  __syncthreads();
+  uint32_t U = 0u;
+#pragma unroll
+  for (int w = 0; w < kStWaves; ++w) U |= wmask[w];
+  U = __builtin_amdgcn_readfirstlane(U);
+
+  const __amdgpu_buffer_rsrc_t ra = make_rsrc(p.a, p.a_bytes);
+  const __amdgpu_buffer_rsrc_t rb = make_rsrc(p.b, p.b_bytes);
+  const T* B = reinterpret_cast<const T*>(p.b);
+
+  // weights of offset f -> registers (WCH 16-byte chunks per thread)
+  auto load_w = [&](int f, u32x4 (&wv)[WCH]) {
+#pragma unroll
+    for (int j = 0; j < WCH; ++j) {
+      const int i = tid + j * NT;  // chunk id: n * CHUNKS + c
+      const int n = i / CHUNKS, c = i % CHUNKS;
+      const bool ok = f >= 0 && i < NCOL * CHUNKS && n < p.nout;
+      uint32_t off = kBufOOB;
+      if (ok) off = (uint32_t)(((int64_t)f * p.b_fs + (int64_t)n * p.b_ns + c * E) * (int64_t)sizeof(T));
+      wv[j] = bld16(rb, off);
+    }
+  };
+  auto store_w = [&](int buf, const u32x4 (&wv)[WCH]) {
+    char* base = wlds + buf * STAGE;
+#pragma unroll
+    for (int j = 0; j < WCH; ++j) {
+      const int i = tid + j * NT;
+      if (i < NCOL * CHUNKS) {
+        const int n = i / CHUNKS, c = i % CHUNKS;
+        *reinterpret_cast<u32x4*>(base + (n * CHUNKS + (c ^ (n & SWZ))) * 16) = wv[j];
+      }
+    }
+  };
+  // pair index of row lr for offset f (every lane group holds its row's copy)
+  auto load_idx = [&](int f) -> int {
+    if (f < 0 || !rvalid) return -1;
+    return p.pairs ? p.pairs[(int64_t)f * p.pair_ld + col] : col;
+  };
+  // gathered operand: lane (lr, lg) holds row lr, reduction elements
+  // kc*CH + lg*E .. + E of each chunk kc
+  auto gather = [&](int idx, u32x4 (&af)[KC]) {
+#pragma unroll
+    for (int kc = 0; kc < KC; ++kc) {
+      const int k = kc * CH + lg * E;
+      const uint32_t off = idx >= 0 ? (uint32_t)(((int64_t)idx * p.lda + k) * (int64_t)sizeof(T)) : kBufOOB;
+      af[kc] = bld16(ra, off);
+    }
+  };
+  auto next_bit = [](uint32_t u, int f) -> int {  // lowest set bit above f (f = -1: lowest)
+    const uint32_t rest = f >= 31 ? 0u : (u & (0xffffffffu << (f + 1)));
+    return rest ? __builtin_ctz(rest) : -1;
+  };
+
+  f32x4 acc[NB];
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  // ---- pipeline over the block's offsets f0 < f1 < f2 < ...
+  const int f0 = next_bit(U, -1);
+  if (f0 >= 0) {
+    const int f1 = next_bit(U, f0);
+    u32x4 wv[WCH];
+    load_w(f0, wv);
+    int i0 = load_idx(f0);
+    int i1 = load_idx(f1);
+    u32x4 a_cur[KC];
+    gather(i0, a_cur);
+    store_w(0, wv);
+    load_w(f1, wv);  // weights of f1, stored during f0's turn
+    __syncthreads();
+    int fc = f0, fn = f1, buf = 0;
+    while (fc >= 0) {
+      const int fnn = fn >= 0 ? next_bit(U, fn) : -1;
+      // rows of the next offset and pair indices of the one after
+      u32x4 a_nxt[KC];
+      gather(i1, a_nxt);
+      const int i2 = load_idx(fnn);
+      if ((km >> fc) & 1u) {
+        const char* wb = wlds + buf * STAGE;
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc) {
+#pragma unroll
+          for (int nb = 0; nb < NB; ++nb) {
+            const int n = nb * 16 + lr;
+            const int c = kc * (CH / E) + lg;
+            const u32x4 bfr = *reinterpret_cast<const u32x4*>(wb + (n * CHUNKS + (c ^ (n & SWZ))) * 16);
+            MM::step(acc[nb], a_cur[kc], bfr);
+          }
+        }
+      }
+      if (fn >= 0) store_w(buf ^ 1, wv);
+      if (fnn >= 0) load_w(fnn, wv);
+      __syncthreads();  // next stage's weights are in LDS; this stage's reads are done
+#pragma unroll
+      for (int kc = 0; kc < KC; ++kc) a_cur[kc] = a_nxt[kc];
+      i1 = i2;
+      fc = fn;
+      fn = fnn;
+      buf ^= 1;
+    }
+  }
+
+  // ---- epilogue: accumulator element i of column block nb is row 4*lg + i,
+  // column nb*16 + lr
+  if (!live) return;
+  OutT* Cout = reinterpret_cast<OutT*>(p.c);
+  const OutT* D = reinterpret_cast<const OutT*>(p.d);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int rr = 4 * lg + i;
+    const int o = __shfl(orow, rr);
+    if (row0 + rr < p.m) {
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        const int n = nb * 16 + lr;
+        if (n < p.nout) {
+          float v = acc[nb][i];
+          if (p.scale) v *= p.scale[n];
+          v *= p.alpha;
+          if (p.beta != 0.f) {
+            const float src = p.d_is_bias ? to_f32(D[n]) : to_f32(D[(int64_t)o * p.ldd + n]);
+            v += p.beta * src;
+          }
+          v = apply_act(v, p.act_type, p.act_alpha);
+          Cout[(int64_t)o * p.ldc + n] = from_f32<OutT>(v);
+        }
+      }
+    }
+  }
+}
+
+// Launches the streamed-weights kernel when it applies (n-major weights, the
+// reduction a multiple of the MFMA chunk with at most 4 chunks, at most 128
+// output columns); *used = false otherwise.
+template <typename T, typename OutT>
+static hipError_t launch_stream(const RowGatherArgs& a, hipStream_t s, bool* used) {
+  *used = false;
+  constexpr int CH = Mma<T>::kChunk;
+  if (a.b_kmajor || a.pair_col == nullptr || a.kred % CH != 0 || a.kred / CH > 4 || a.nout > 128 || a.nout < 1)
+    return hipSuccess;
+  const int kc = a.kred / CH;
+  const int nb = a.nout <= 32 ? 2 : (a.nout <= 64 ? 4 : 8);
+  if (2 * nb * 16 * a.kred * (int)sizeof(T) > kStLdsBytes) return hipSuccess;
+  const int ntiles = (a.m + kMaskTile - 1) / kMaskTile;
+  if (ntiles == 0) {
+    *used = true;
+    return hipSuccess;
+  }
+  dim3 grid((unsigned)((ntiles + kStWaves - 1) / kStWaves)), block(64 * kStWaves);
+  *used = true;
+#define CS_ST(KC_, NB_) hipLaunchKernelGGL((rowgather_stream_kernel<T, OutT, KC_, NB_>), grid, block, 0, s, a, ntiles)
+  switch (kc * 16 + nb) {
+    case 1 * 16 + 2: CS_ST(1, 2); break;
+    case 1 * 16 + 4: CS_ST(1, 4); break;
+    case 1 * 16 + 8: CS_ST(1, 8); break;
+    case 2 * 16 + 2: CS_ST(2, 2); break;
+    case 2 * 16 + 4: CS_ST(2, 4); break;
+    case 2 * 16 + 8: CS_ST(2, 8); break;
+    case 4 * 16 + 2: CS_ST(4, 2); break;
+    case 4 * 16 + 4: CS_ST(4, 4); break;
+    case 4 * 16 + 8: CS_ST(4, 8); break;
+    default: *used = false; return hipSuccess;
+  }
+#undef CS_ST
+  return hipGetLastError();
+}
+
+}  // namespace cs

@@ -263,3 +263,45 @@ def test_dgrad_block_kernel_path(C, K):
     exp = ref.conv_dgrad(quantize(c.dO, "f16"), quantize(c.W, "f16"), prob.pair_bwd, prob.mask_sorted,
                          prob.argsort, 16, reverse_mask=True)
     assert _rel(din.double().cpu().numpy(), quantize(exp, "f16")) <= TOL["f16"]
+
+
+@pytest.mark.parametrize("dt", ["f16", "bf16"])
+def test_wide_layers_streamed_weights(dt):
+    """128 -> 128 channels: the weights do not fit the LDS for all output
+    columns, so forward and dgrad run the streamed-weights kernel."""
+    c = subm_case(2500, 128, 128, seed=17)
+    prob = c.prob
+    print(f"[wide] dt={dt}", flush=True)
+    out, mo, desp = run_fwd(c, dt)
+    exp, tm = ref.conv_forward(quantize(c.X, dt), quantize(c.W, dt), prob.pair_fwd, prob.mask_sorted,
+                               prob.argsort, 16)
+    assert np.array_equal(mo, tm)
+    assert _rel(out, quantize(exp, dt)) <= TOL[dt]
+    desp = desp_for(1, dt, c.C, c.K)
+    din = torch.zeros((prob.n, c.C), dtype=TORCH_DT[dt], device="cuda")
+    p = conv_params(1, desp, din, to_dev(c.W, dt), to_dev(c.dO, dt), prob.pair_bwd, prob.mask_sorted,
+                    prob.argsort, reverse_mask=True)
+    ConvMainUnitTest.implicit_gemm2(p)
+    torch.cuda.synchronize()
+    exp = ref.conv_dgrad(quantize(c.dO, dt), quantize(c.W, dt), prob.pair_bwd, prob.mask_sorted, prob.argsort,
+                         16, reverse_mask=True)
+    assert _rel(din.double().cpu().numpy(), quantize(exp, dt)) <= TOL[dt]
+
+
+def test_streamed_equals_weight_stationary():
+    """The streamed-weights kernel (block_n 128) and the weight-stationary one
+    (block_n 64) give the same bits: each row sums its offsets in order."""
+    c = subm_case(2000, 64, 64, seed=23)
+    prob = c.prob
+    outs = []
+    for bn in (64, 128):
+        desp = [d for d in ConvMainUnitTest.get_all_conv_algo_desp()
+                if int(d.op_type) == 0 and d.dtype_input == DT["f16"] and d.dtype_output == DT["f16"]
+                and d.tile_shape[1] == bn][0]
+        out = torch.zeros((prob.n, c.K), dtype=torch.float16, device="cuda")
+        p = conv_params(0, desp, to_dev(c.X, "f16"), to_dev(c.W, "f16"), out, prob.pair_fwd, prob.mask_sorted,
+                        prob.argsort)
+        ConvMainUnitTest.implicit_gemm2(p)
+        torch.cuda.synchronize()
+        outs.append(out.clone())
+    assert torch.equal(outs[0], outs[1])
