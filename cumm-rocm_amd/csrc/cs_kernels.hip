@@ -172,20 +172,23 @@ template <> struct WgRows<float> { static constexpr int value = 2; };
 
 // Partial products over one piece of offset f's pair list: rows a[list_a] and
 // b[list_b], 64 x 64 output tile, FP32 partial per piece.
-template <typename T>
+// TM x TN output tile per block (64 or 128 each): a wider tile gathers every
+// row once for more output channels.  Four waves in a 2 x 2 grid.
+template <typename T, int TM, int TN>
 __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs p) {
   using MM = Mma<T>;
   constexpr int E = MM::kElems;
   constexpr int CH = MM::kChunk;
   constexpr int R = kWgradRows;
-  constexpr int TS = kWgradTile;
   constexpr int PITCH = R * (int)sizeof(T) + 16;
-  constexpr int CHUNKS = TS / E;          // 16-byte chunks per 64 channels
-  constexpr int CPT = (CHUNKS + 3) / 4;   // chunks per thread per operand and row
-  constexpr int RPT = WgRows<T>::value;   // rows per thread per chunk
+  constexpr int CPA = (TM / E + 3) / 4;   // A chunks per thread and row (TM channels)
+  constexpr int CPB = (TN / E + 3) / 4;   // B chunks per thread and row
+  // rows per thread and chunk: fewer when the tile is wide, to bound registers
+  constexpr int RPT = (TM + TN > 128) ? (sizeof(T) == 4 ? 1 : 2) : WgRows<T>::value;
   constexpr int CROWS = RPT * R;          // rows per chunk
-  __shared__ __attribute__((aligned(16))) char lds_a[TS * PITCH];
-  __shared__ __attribute__((aligned(16))) char lds_b[TS * PITCH];
+  constexpr int FA = TM / 32, FB = TN / 32;  // 16-wide fragments per wave
+  __shared__ __attribute__((aligned(16))) char lds_a[TM * PITCH];
+  __shared__ __attribute__((aligned(16))) char lds_b[TN * PITCH];
 
   const int f = blockIdx.y;
   if (!((p.mask_filter >> f) & 1u)) return;
@@ -193,24 +196,24 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs p) {
   const int nb = wgrad_pieces(count, p.max_blocks, p.min_rows);
   const int piece = blockIdx.z;
   if (piece >= nb) return;
-  const int tiles_n = (p.N + TS - 1) / TS;
-  const int m0 = (blockIdx.x / tiles_n) * TS;
-  const int nn0 = (blockIdx.x % tiles_n) * TS;
+  const int tiles_n = (p.N + TN - 1) / TN;
+  const int m0 = (blockIdx.x / tiles_n) * TM;
+  const int nn0 = (blockIdx.x % tiles_n) * TN;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 15, lg = lane >> 4;
-  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  const int wm = (wave & 1) * (TM / 2), wn = (wave >> 1) * (TN / 2);
   const int srow = tid & 63, sq = tid >> 6;
   const int per = (count + nb - 1) / nb;
   const int q0 = piece * per;
   const int nrows_blk = min(per, count - q0);
   const int nchunks = (nrows_blk + CROWS - 1) / CROWS;
 
-  f32x4 acc[2][2];
+  f32x4 acc[FA][FB];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < FA; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < FB; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
   // All loads are unconditional buffer loads (out-of-range offsets read 0).
   const __amdgpu_buffer_rsrc_t rA = make_rsrc(p.a, p.a_bytes);
@@ -232,16 +235,20 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs p) {
   int ra0[RPT], rb0[RPT];
   ld_idx(0, ra0, rb0);
   for (int j = 0; j < nchunks; ++j) {
-    u32x4 va[RPT][CPT], vb[RPT][CPT];
+    u32x4 va[RPT][CPA], vb[RPT][CPB];
 #pragma unroll
     for (int st = 0; st < RPT; ++st) {
       const bool ok = j * CROWS + st * R + srow < nrows_blk;
 #pragma unroll
-      for (int i = 0; i < CPT; ++i) {
+      for (int i = 0; i < CPA; ++i) {
         const int ch = (sq + 4 * i) * E;
-        const bool oka = ok && ch < TS && m0 + ch < p.M;
-        const bool okb = ok && ch < TS && nn0 + ch < p.N;
+        const bool oka = ok && ch < TM && m0 + ch < p.M;
         va[st][i] = bld16(rA, oka ? (uint32_t)(((int64_t)ra0[st] * p.lda + m0 + ch) * (int)sizeof(T)) : kBufOOB);
+      }
+#pragma unroll
+      for (int i = 0; i < CPB; ++i) {
+        const int ch = (sq + 4 * i) * E;
+        const bool okb = ok && ch < TN && nn0 + ch < p.N;
         vb[st][i] = bld16(rB, okb ? (uint32_t)(((int64_t)rb0[st] * p.ldb + nn0 + ch) * (int)sizeof(T)) : kBufOOB);
       }
     }
@@ -251,27 +258,29 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs p) {
     for (int st = 0; st < RPT; ++st) {
       __syncthreads();  // previous stage's MFMAs are done with the tiles
 #pragma unroll
-      for (int i = 0; i < CPT; ++i) {
+      for (int i = 0; i < CPA; ++i) {
         const int ch = (sq + 4 * i) * E;
-        if (ch < TS) {
-          TrStore<T>::put(lds_a, PITCH, ch, srow, va[st][i]);
-          TrStore<T>::put(lds_b, PITCH, ch, srow, vb[st][i]);
-        }
+        if (ch < TM) TrStore<T>::put(lds_a, PITCH, ch, srow, va[st][i]);
+      }
+#pragma unroll
+      for (int i = 0; i < CPB; ++i) {
+        const int ch = (sq + 4 * i) * E;
+        if (ch < TN) TrStore<T>::put(lds_b, PITCH, ch, srow, vb[st][i]);
       }
       __syncthreads();
 #pragma unroll
       for (int kc = 0; kc < R; kc += CH) {
-        u32x4 af[2], bfr[2];
+        u32x4 af[FA], bfr[FB];
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < FA; ++i)
           af[i] = *reinterpret_cast<const u32x4*>(lds_a + (wm + 16 * i + lr) * PITCH + (kc + lg * E) * (int)sizeof(T));
 #pragma unroll
-        for (int jj = 0; jj < 2; ++jj)
+        for (int jj = 0; jj < FB; ++jj)
           bfr[jj] = *reinterpret_cast<const u32x4*>(lds_b + (wn + 16 * jj + lr) * PITCH + (kc + lg * E) * (int)sizeof(T));
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < FA; ++i)
 #pragma unroll
-          for (int jj = 0; jj < 2; ++jj) MM::step(acc[i][jj], af[i], bfr[jj]);
+          for (int jj = 0; jj < FB; ++jj) MM::step(acc[i][jj], af[i], bfr[jj]);
       }
     }
 #pragma unroll
@@ -283,9 +292,9 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs p) {
 
   float* out = p.partial + ((int64_t)f * p.max_blocks + piece) * (int64_t)p.M * p.N;
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < FA; ++i)
 #pragma unroll
-    for (int jj = 0; jj < 2; ++jj)
+    for (int jj = 0; jj < FB; ++jj)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int mm = m0 + wm + 16 * i + 4 * lg + e;
@@ -431,6 +440,16 @@ hipError_t launch_rowgather(int in_dtype, int out_dtype, int block_n, const RowG
   }
 }
 
+template <typename T>
+static hipError_t launch_wgrad_t(int tm, int tn, dim3 grid, const WgradArgs& a, hipStream_t s) {
+  dim3 block(256, 1, 1);
+  if (tm == 64 && tn == 64) hipLaunchKernelGGL((wgrad_kernel<T, 64, 64>), grid, block, 0, s, a);
+  else if (tm == 64) hipLaunchKernelGGL((wgrad_kernel<T, 64, 128>), grid, block, 0, s, a);
+  else if (tn == 64) hipLaunchKernelGGL((wgrad_kernel<T, 128, 64>), grid, block, 0, s, a);
+  else hipLaunchKernelGGL((wgrad_kernel<T, 128, 128>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_wgrad(int in_dtype, const WgradArgs& a, hipStream_t s) {
   if (a.max_blocks < 1 || a.max_blocks > kWgradMaxBlocks || a.min_rows < 1) return hipErrorInvalidValue;
   hipError_t e;
@@ -446,18 +465,19 @@ hipError_t launch_wgrad(int in_dtype, const WgradArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(wgrad_list_kernel, dim3(nblk, a.kv), dim3(256), 0, s, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
-  const int tiles = ((a.M + kWgradTile - 1) / kWgradTile) * ((a.N + kWgradTile - 1) / kWgradTile);
+  // 128-wide tiles where a dimension exceeds 64: each row is gathered once per tile
+  const int tm = a.M > 64 ? 128 : 64, tn = a.N > 64 ? 128 : 64;
+  const int tiles = ((a.M + tm - 1) / tm) * ((a.N + tn - 1) / tn);
   dim3 grid(tiles, a.kv, a.max_blocks);
   dim3 block(256, 1, 1);
   if (tiles == 0) return hipSuccess;
   switch (in_dtype) {
-    case kF32: hipLaunchKernelGGL(wgrad_kernel<float>, grid, block, 0, s, a); break;
-    case kF16: hipLaunchKernelGGL(wgrad_kernel<_Float16>, grid, block, 0, s, a); break;
-    case kBF16: hipLaunchKernelGGL(wgrad_kernel<bf16>, grid, block, 0, s, a); break;
-    case kE4M3: hipLaunchKernelGGL(wgrad_kernel<e4m3>, grid, block, 0, s, a); break;
+    case kF32: return launch_wgrad_t<float>(tm, tn, grid, a, s);
+    case kF16: return launch_wgrad_t<_Float16>(tm, tn, grid, a, s);
+    case kBF16: return launch_wgrad_t<bf16>(tm, tn, grid, a, s);
+    case kE4M3: return launch_wgrad_t<e4m3>(tm, tn, grid, a, s);
     default: return hipErrorInvalidValue;
   }
-  return hipGetLastError();
 }
 
 template <typename OutT>

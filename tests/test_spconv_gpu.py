@@ -305,3 +305,28 @@ def test_streamed_equals_weight_stationary():
         torch.cuda.synchronize()
         outs.append(out.clone())
     assert torch.equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("C,K,dt", [(128, 128, "f16"), (64, 128, "bf16"), (128, 64, "f32"), (128, 128, "e4m3")])
+def test_wgrad_wide_tiles(C, K, dt):
+    """wgrad with 128-wide output tiles (each gathered row serves 128 channels)."""
+    c = subm_case(2000, C, K, seed=C + 2 * K)
+    prob = c.prob
+    print(f"[wgrad-wide] C={C} K={K} dt={dt}", flush=True)
+    _, mo, fdesp = run_fwd(c, "f32")
+    out_dt = "f32" if dt == "e4m3" else dt
+    desp = desp_for(2, dt, C, K, out_dt)
+    dw = torch.zeros((K, prob.kv, C), dtype=TORCH_DT[out_dt], device="cuda")
+    if dt == "e4m3":
+        xq, x64 = _e4m3_quant(c.X, 1.0)
+        dq, d64 = _e4m3_quant(c.dO, 1.0)
+        xin, dout = xq.cuda(), dq.cuda()
+    else:
+        xin, dout = to_dev(c.X, dt), to_dev(c.dO, dt)
+        x64, d64 = quantize(c.X, dt), quantize(c.dO, dt)
+    p = conv_params(2, desp, xin, dw, dout, prob.pair_fwd, mo, prob.argsort, mask_width=16)
+    ConvMainUnitTest.implicit_gemm2(p)
+    torch.cuda.synchronize()
+    exp = ref.conv_wgrad(x64, d64, prob.pair_fwd, mo, prob.argsort, 16)
+    tol = 1e-4 if dt == "e4m3" else max(TOL[dt], TOL[out_dt]) * 2
+    assert _rel(dw.double().cpu().numpy(), quantize(exp, out_dt)) <= tol
