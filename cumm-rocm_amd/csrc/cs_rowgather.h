@@ -224,9 +224,9 @@ __global__ __launch_bounds__(256) void rowgather_kernel(RowGatherArgs p) {
 
 // Kernel choice.  block_n (the algo's tile_shape[1]) of 16 / 32 / 64 caps the
 // weight-stationary kernel's column slice; with 64 that kernel runs when it
-// needs at most two column slices and the streamed-weights kernel when it
-// would need more (every slice gathers the input rows again); 128 asks for
-// the streamed-weights kernel.  The block kernel covers the rest.
+// needs at most two column slices (four for f32) and the streamed-weights
+// kernel when it would need more (every slice gathers the input rows again);
+// 128 asks for the streamed-weights kernel.  The block kernel covers the rest.
 template <typename T>
 static int ws_slices(const RowGatherArgs& a) {
   int ncol = std::min(ws_ncol<T>(a.kv, a.kred), 64);
@@ -239,7 +239,8 @@ static hipError_t launch_rg(const RowGatherArgs& a, int block_n, hipStream_t s) 
   if (block_n != 16 && block_n != 32 && block_n != 64 && block_n != 128) return hipErrorInvalidValue;
   bool used = false;
   hipError_t e;
-  const bool prefer_stream = block_n == 128 || (block_n == 64 && ws_slices<T>(a) > 2);
+  // measured on MI355X: two slices beat streaming for 1- and 2-byte types, four for f32
+  const bool prefer_stream = block_n == 128 || (block_n == 64 && ws_slices<T>(a) > (sizeof(T) == 4 ? 4 : 2));
   if (!prefer_stream) {
     e = launch_ws<T, OutT>(a, block_n, s, &used);
     if (used || e != hipSuccess) return e;
