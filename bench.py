@@ -46,10 +46,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", choices=("c2", "c3", "c4", "c5"), default="c2",
+    ap.add_argument("--config", choices=("c2", "c3", "c4", "c5", "f3"), default="c2",
                     help="c2: 120k voxels 64->64 (default); c3: precision form, 200k voxels 128->128; "
                          "c4: 32 clouds, strided 32->64 + subm 64->128, grouped launches under a HIP graph; "
-                         "c5: serving, Poisson 20-200 frames/s, 4-layer BF16 stack")
+                         "c5: serving, Poisson 20-200 frames/s, 4-layer BF16 stack; "
+                         "f3: int8-inference forward and the gather / scatter-add glue on the c2 scene")
     ap.add_argument("--rates", default="20,50,100,200", help="c5 arrival rates (frames/s)")
     ap.add_argument("--seconds", type=float, default=3.0, help="c5 traffic per rate")
     ap.add_argument("--voxels", type=int, default=None)
@@ -62,7 +63,7 @@ def parse():
     ap.add_argument("--copy-bytes", type=int, default=0)
     args = ap.parse_args()
     dflt = {"c2": (120000, 64, "f32,f16,bf16"), "c3": (200000, 128, "f32,bf16,f16,e4m3"),
-            "c4": (0, 0, "f16"), "c5": (0, 0, "bf16")}[args.config]
+            "c4": (0, 0, "f16"), "c5": (0, 0, "bf16"), "f3": (120000, 64, "i8")}[args.config]
     args.voxels = args.voxels or dflt[0]
     args.channels = args.channels or dflt[1]
     args.dtypes = args.dtypes or dflt[2]
@@ -483,6 +484,108 @@ def run_serving(args):
     print(json.dumps(line))
 
 
+def run_f3(args):
+    """int8-inference forward (int8 features / weights, int32 accumulation,
+    float scale + bias epilogue, int8 out) and the gather-GEMM-scatter glue of
+    spconv's native algorithm (per kernel offset: gather the offset's input
+    rows, scatter-add its FP32 output rows), on the c2 scene."""
+    import torch
+    from cumm_amd import pointcloud as pc
+    from cumm_amd import tensorview as tv
+    from cumm_amd.conv.main import ConvMainUnitTest, select_conv_algo
+    from cumm_amd.gemm.gather import GatherAll, ScatterAll
+    from cumm_amd.tensorview.gemm import ConvOpType, ConvParams
+
+    coords = pc.plane_scene(args.voxels, seed=args.seed)
+    prob = pc.build_subm(coords, pc.GRID_KITTI)
+    n, C, K, kv = prob.n, args.channels, args.channels, prob.kv
+    P = prob.num_pairs
+    rng = np.random.default_rng(args.seed + 1)
+    dev = "cuda"
+    stream = torch.cuda.current_stream()
+    sid = stream.cuda_stream
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(args.steps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / args.steps * 1e-3
+
+    # ---- int8 inference forward
+    xq = torch.from_numpy(rng.integers(-127, 128, (n, C)).astype(np.int8)).to(dev)
+    wq = torch.from_numpy(rng.integers(-127, 128, (K, kv, C)).astype(np.int8)).to(dev)
+    out = torch.empty((n, K), dtype=torch.int8, device=dev)
+    p = ConvParams(3, ConvOpType.Forward)
+    p.conv_algo_desp = select_conv_algo(0, tv.int8, tv.int8, tv.int8, C, K)
+    p.input, p.weight, p.output = tv.from_torch(xq), tv.from_torch(wq), tv.from_torch(out)
+    p.indices = tv.from_torch(torch.from_numpy(prob.pair_fwd).to(dev))
+    p.mask = tv.from_torch(torch.from_numpy(prob.mask_sorted.view(np.int32)).to(dev))
+    p.mask_argsort = tv.from_torch(torch.from_numpy(prob.argsort).to(dev))
+    p.mask_output = tv.from_torch(torch.zeros(((n + 15) // 16,), dtype=torch.int32, device=dev))
+    p.scale = tv.from_torch(torch.full((K,), 1e-4, device=dev))
+    p.bias = tv.from_torch(torch.zeros((K,), device=dev))
+    p.stream = sid
+    t_i8 = timed(lambda: ConvMainUnitTest.implicit_gemm2(p))
+    b_i8 = op_bytes("fwd", n, kv, C, K, 1, 1)
+    ops_i8 = 2 * P * C * K
+    log(f"int8 fwd: {t_i8 * 1e6:.1f} us  {ops_i8 / t_i8 / 1e12:.1f} TOP/s  {b_i8 / t_i8 / 1e9:.0f} GB/s")
+
+    # ---- gather / scatter-add glue, per kernel offset (compact pairs)
+    ins, outs = [], []
+    for f in range(kv):
+        o = np.nonzero(prob.pair_fwd[f] >= 0)[0].astype(np.int32)
+        ins.append(torch.from_numpy(prob.pair_fwd[f][o].astype(np.int32)).to(dev))
+        outs.append(torch.from_numpy(o).to(dev))
+    feat = torch.from_numpy(rng.standard_normal((n, C)).astype(np.float16)).to(dev)
+    buf = torch.empty((int(max(t.numel() for t in ins)), C), dtype=torch.float16, device=dev)
+    part = torch.from_numpy(rng.standard_normal((buf.shape[0], K)).astype(np.float32)).to(dev)
+    acc = torch.zeros((n, K), dtype=torch.float32, device=dev)
+    gp, sp = (8, 2048, 16, 256), (8, 2048, 16, 256)
+    ga, sa = GatherAll(), ScatterAll()
+    tvs = [(tv.from_torch(buf[:i.numel()]), tv.from_torch(i), tv.from_torch(part[:i.numel()]), tv.from_torch(o))
+           for i, o in zip(ins, outs)]
+    tfeat, tacc = tv.from_torch(feat), tv.from_torch(acc)
+
+    def gather_all():
+        for b, i, _, _ in tvs:
+            ga.gather(b, tfeat, i, *gp, sid)
+
+    def scatter_all():
+        for _, _, pr, o in tvs:
+            sa.scatter(tacc, pr, o, *sp, sid)
+
+    t_g = timed(gather_all)
+    t_s = timed(scatter_all)
+    b_g = P * C * 2 * 2 + P * 4               # read + write each gathered row, indices
+    b_s = P * K * 4 * 3 + P * 4               # read partial, read + write target, indices
+    log(f"gather: {t_g * 1e6:.1f} us {b_g / t_g / 1e9:.0f} GB/s; scatter-add: {t_s * 1e6:.1f} us "
+        f"{b_s / t_s / 1e9:.0f} GB/s ({kv} launches each)")
+    workload = f"subm3x3x3 forward, {n} voxels on 8 planes in 41x1600x1408, {C}->{K} ch"
+    line = {
+        "metric": "int8-inference sparse-conv TOP/s; gather / scatter-add GB/s",
+        "value": ops_i8 / t_i8 / 1e12, "unit": "TOP/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_i8 * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "i8", "data": "synthetic",
+        "config": {"workload": workload, "voxels": n, "pairs": P, "channels_in": C, "channels_out": K,
+                   "desp": repr(p.conv_algo_desp)},
+        "roofline": {"kernel": "rowgather_ws_kernel<int8_t> (fwd)", "bound": "hbm", "achieved": b_i8 / t_i8 / 1e9,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": b_i8 / t_i8 / HBM_PEAK, "traffic": None,
+                     "algorithmic_bytes": b_i8, "avg_us": t_i8 * 1e6,
+                     "mfma_frac": ops_i8 / t_i8 / 5.0e15},
+        "glue": {"gather_us": t_g * 1e6, "gather_GB/s": b_g / t_g / 1e9, "gather_frac_hbm": b_g / t_g / HBM_PEAK,
+                 "scatter_us": t_s * 1e6, "scatter_GB/s": b_s / t_s / 1e9,
+                 "scatter_frac_hbm": b_s / t_s / HBM_PEAK, "launches": kv, "rows": P,
+                 "gather_dtype": "f16", "scatter_dtype": "f32"},
+    }
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -492,6 +595,8 @@ def main():
         return run_batched(args)
     if args.config == "c5":
         return run_serving(args)
+    if args.config == "f3":
+        return run_f3(args)
     import torch
     from cumm_amd import pointcloud as pc
 

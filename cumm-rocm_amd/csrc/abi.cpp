@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <climits>
 #include <cstring>
 #include <sstream>
@@ -255,9 +256,9 @@ bool conv_supported(const cs_conv_algo_desp& d, int64_t m, int64_t n, int64_t k,
 }
 
 // ------------------------------------------------------------------ catalogue
-int chunk_elems(int dt) { return dt == 0 ? 16 : dt == 14 ? 64 : 32; }
+int chunk_elems(int dt) { return dt == 0 ? 16 : (dt == 14 || dt == 3) ? 64 : 32; }
 int vec_elems(int dt) { return 16 / dtype_bytes(dt); }
-int mfma_k(int dt) { return dt == 0 ? 4 : 32; }
+int mfma_k(int dt) { return dt == 0 ? 4 : dt == 3 ? 64 : 32; }
 
 cs_gemm_algo_desp base_gemm_desp() {
   cs_gemm_algo_desp d;
@@ -367,6 +368,16 @@ const std::vector<cs_conv_algo_desp>& conv_catalogue() {
       // wgrad: a = dout, b = input, c = dW; trans (T, F, F)
       v.push_back(conv_desp(2, wgrad_desp(c.in, c.w, c.out, cs::kMaskTile)));
     }
+    // int8 inference forward: int8 input / weight / output, int32 accumulation,
+    // float epilogue act(scale * acc + bias + beta * output_add)
+    for (int bn : {64, 32, 16, 128}) {
+      cs_gemm_algo_desp g = rowgather_desp(3, 3, 3, cs::kMaskTile, bn, 0, 1, 0);
+      g.dacc = 1;
+      g.dcomp = 0;
+      cs_conv_algo_desp d = conv_desp(0, g);
+      d.is_int8_inference = 1;
+      v.push_back(d);
+    }
     return v;
   }();
   return cat;
@@ -399,8 +410,19 @@ const std::vector<cs_gemm_algo_desp>& gemm_catalogue() {
   return cat;
 }
 
-bool same_kernel(const cs_conv_algo_desp& a, const cs_conv_algo_desp& b) {
-  return conv_repr(a) == conv_repr(b);
+// Kernel identity is the kernel name (ConvAlgoDesp::__repr__); the catalogue's
+// names are built once.
+const cs_conv_algo_desp* find_conv_kernel(const cs_conv_algo_desp& desp) {
+  static const std::vector<std::string> names = [] {
+    std::vector<std::string> v;
+    for (const auto& d : conv_catalogue()) v.push_back(conv_repr(d));
+    return v;
+  }();
+  const std::string want = conv_repr(desp);
+  const auto& catalogue = conv_catalogue();
+  for (size_t i = 0; i < catalogue.size(); ++i)
+    if (names[i] == want) return &catalogue[i];
+  return nullptr;
 }
 
 // ------------------------------------------------------------------ workspace
@@ -507,6 +529,12 @@ ConvPlan plan_conv(const cs_conv_params& p) {
   } else {
     CS_RT(!empty(p.bias) && !empty(p.scale), "int8 inference must have both scale and bias");
   }
+  if (!empty(p.output_add)) {
+    // nvrtc_conv_template (cumm/conv/nvrtc_code.py:84-89)
+    bool same = p.output_add.dtype == p.output.dtype && p.output_add.ndim == p.output.ndim;
+    for (int i = 0; same && i < p.output.ndim; ++i) same = p.output_add.shape[i] == p.output.shape[i];
+    CS_RT(same, "output and output_add must have same dtype and shape");
+  }
   if (desp.op_type == 2) CS_RT(p.mask_width > 0 && p.mask_width % desp.gemm.tile_shape[2] == 0, "error");
   CS_RT(!empty(p.indices), "error");
   CS_RT(!empty(p.mask), "error");
@@ -532,9 +560,7 @@ ConvPlan plan_conv(const cs_conv_params& p) {
   CS_RT(conv_supported(desp, plan.m, plan.n, plan.k, C, K, p.mask_width), "error");
 
   // kernel selection (conv_select_helper, cumm/conv/main.py:392-465): exact desp match
-  const auto& catalogue = conv_catalogue();
-  for (const auto& d : catalogue)
-    if (same_kernel(d, desp)) { plan.kernel = &d; break; }
+  plan.kernel = find_conv_kernel(desp);
   CS_ARG(plan.kernel != nullptr, "Can't Found Algorithm for params: ", conv_repr(desp));
 
   // Checks the generated kernels leave to the caller; here they guard memory safety.
@@ -571,7 +597,15 @@ ConvPlan plan_conv(const cs_conv_params& p) {
   if (!empty(p.bias)) {
     check_device(p.bias, "bias");
     CS_ARG(numel(p.bias) >= (desp.op_type == 0 ? K : C), "bias too short");
-    CS_ARG(p.bias.dtype == dt_written, "bias dtype must equal the dtype of the written tensor");
+    if (desp.is_int8_inference)  // bias is in the compute type (float32)
+      CS_ARG(p.bias.dtype == desp.gemm.dcomp, "int8 inference bias must be float32");
+    else
+      CS_ARG(p.bias.dtype == dt_written, "bias dtype must equal the dtype of the written tensor");
+    check_dense(p.bias, "bias");
+  }
+  if (!empty(p.output_add)) {
+    check_device(p.output_add, "output_add");
+    check_rows(p.output_add, "output_add");
   }
   // 16-byte operand loads: row strides and bases of the gathered operands.
   const int vec_a = vec_elems(desp.gemm.dtype_a), vec_b = vec_elems(desp.gemm.dtype_b);
@@ -585,8 +619,12 @@ ConvPlan plan_conv(const cs_conv_params& p) {
                aligned16(p.input.data),
            "input and output rows must be 16-byte aligned");
   }
-  if (!empty(p.scale)) CS_ARG(p.scale.dtype == 0 && numel(p.scale) >= (desp.op_type == 1 ? C : K),
-                              "scale must be float32 per output channel");
+  if (!empty(p.scale)) {
+    CS_ARG(p.scale.dtype == 0 && numel(p.scale) >= (desp.op_type == 1 ? C : K),
+           "scale must be float32 per output channel");
+    check_device(p.scale, "scale");
+    check_dense(p.scale, "scale");
+  }
 
   plan.N = N; plan.C = C; plan.K = K; plan.kv = kv;
   if (desp.op_type == 1) {
@@ -655,6 +693,14 @@ void run_conv(const cs_conv_params& p) {
     a.act_alpha = p.act_alpha;
     a.act_beta = p.act_beta;
     a.act_type = p.act_type;
+    if (desp.is_int8_inference) {
+      // source = output_add, else the output itself (cumm/conv/main.py:604-610)
+      const cs_tensor& srct = empty(p.output_add) ? p.output : p.output_add;
+      a.bias_f = reinterpret_cast<const float*>(p.bias.data);
+      a.d = srct.data;
+      a.ldd = row_stride(srct);
+      a.d_is_bias = 0;
+    }
     CS_HIP(cs::launch_rowgather(src.dtype, dst.dtype, desp.gemm.tile_shape[1], a, s));
   } else {
     cs::WgradArgs w;
@@ -781,10 +827,15 @@ GemmPlan plan_gemm(const cs_gemm_params& p) {
   if (g.d.ndim == 1) CS_RT(dim(g.d, 0) == g.n, "d must be a valid bias");
   CS_ARG(p.c.dtype == g.d.dtype, "d dtype must equal to c");
 
+  static const std::vector<std::string> names = [] {
+    std::vector<std::string> v;
+    for (const auto& d : gemm_catalogue()) v.push_back(gemm_repr(d));
+    return v;
+  }();
   const auto& catalogue = gemm_catalogue();
   const std::string want = gemm_repr(desp);
-  for (const auto& d : catalogue)
-    if (gemm_repr(d) == want) { g.kernel = &d; break; }
+  for (size_t i = 0; i < catalogue.size(); ++i)
+    if (names[i] == want) { g.kernel = &catalogue[i]; break; }
   CS_ARG(g.kernel != nullptr, "Can't Found Algorithm for params: ", want);
   CS_ARG(p.a.dtype == desp.dtype_a && p.b.dtype == desp.dtype_b && p.c.dtype == desp.dtype_c,
          "tensor dtypes do not match the algo ", want);
@@ -927,6 +978,96 @@ void run_gemm(const cs_gemm_params& p) {
   CS_HIP(cs::launch_rowgather(p.a.dtype, p.c.dtype, g.kernel->tile_shape[1], a, s));
 }
 
+// ------------------------------------------------------------------ gather / scatter
+// Published parameter sets of GatherAll / ScatterAll (cumm/gemm/gather.py:392-420,
+// 500-530), in the reference's enumeration order: tile k bytes, tile m, bytes
+// per access, threads, keeping sets whose threads * access fit the tile.
+using GsParam = std::array<int32_t, 4>;  // tile_m, tile_k_bytes, bytes_per_access, num_threads
+
+std::vector<GsParam> gs_params(bool scatter) {
+  std::vector<GsParam> out;
+  const std::vector<int> tk_gather = {4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048};
+  const std::vector<int> tk_scatter = {64, 128, 256, 512, 1024, 2048};
+  const std::vector<int> th_gather = {64, 128, 256};
+  const std::vector<int> th_scatter = {128, 256};
+  for (int tk : scatter ? tk_scatter : tk_gather)
+    for (int tm : {1, 2, 4, 8})
+      for (int bpa : {4, 8, 16})
+        for (int nt : scatter ? th_scatter : th_gather) {
+          if (nt * bpa > tk * tm) continue;
+          out.push_back({tm, tk, bpa, nt});
+        }
+  return out;
+}
+
+const std::vector<GsParam>& gather_params() {
+  static const std::vector<GsParam> v = gs_params(false);
+  return v;
+}
+
+const std::vector<GsParam>& scatter_params() {
+  static const std::vector<GsParam> v = gs_params(true);
+  return v;
+}
+
+bool has_param(const std::vector<GsParam>& v, int64_t tm, int64_t tk, int64_t bpa, int64_t nt) {
+  for (const auto& q : v)
+    if (q[0] == tm && q[1] == tk && q[2] == bpa && q[3] == nt) return true;
+  return false;
+}
+
+// Gather::supported / Scatter::supported (cumm/gemm/gather.py:210-219, 345-353)
+bool gs_supported(int64_t bpa, int64_t channel_size, int dtype) {
+  const int isz = dtype_bytes(dtype);
+  return isz > 0 && bpa > 0 && (channel_size * isz) % bpa == 0;
+}
+
+// Shared preamble of Gather::gather / Scatter::scatter (cumm/gemm/gather.py:236-243, 368-375)
+void gs_check(const cs_tensor& out, const cs_tensor& in, const cs_tensor& idx, int64_t bpa) {
+  CS_RT(in.ndim == 2 && out.ndim == 2 && in.shape[1] == out.shape[1], "error");
+  CS_RT(in.dtype == out.dtype, "error");
+  CS_RT(gs_supported(bpa, in.shape[1], in.dtype), "not supported. use supported() to check first");
+  CS_ARG(idx.ndim == 1, "indices must be 1-D");
+  check_index_dtype(idx, "indices");
+  check_device(in, "input");
+  check_device(out, "output");
+  check_device(idx, "indices");
+  check_rows(in, "input");
+  check_rows(out, "output");
+  CS_ARG(idx.shape[0] == 0 || idx.stride[0] == 1, "indices must be contiguous");
+}
+
+void run_gather(const cs_tensor& out, const cs_tensor& in, const cs_tensor& idx, int64_t tm, int64_t tk,
+                int64_t bpa, int64_t nt, uint64_t stream) {
+  CS_RT(has_param(gather_params(), tm, tk, bpa, nt), "can't find a sutable gather");
+  gs_check(out, in, idx, bpa);
+  const int64_t m = idx.shape[0];
+  CS_ARG(out.shape[0] >= m, "output has ", out.shape[0], " rows, indices ", m);
+  const int64_t isz = dtype_bytes(in.dtype);
+  CS_HIP(cs::launch_gather_rows(in.data, out.data, reinterpret_cast<const int32_t*>(idx.data), m, in.shape[0],
+                                in.shape[1] * isz, row_stride(in) * isz, row_stride(out) * isz,
+                                reinterpret_cast<hipStream_t>(stream)));
+}
+
+void run_scatter(const cs_tensor& out, const cs_tensor& in, const cs_tensor& idx, int64_t tm, int64_t tk,
+                 int64_t bpa, int64_t nt, uint64_t stream) {
+  // the scatter catalogue holds float32 kernels only (cumm/gemm/gather.py:515)
+  CS_RT(in.dtype == 0 && has_param(scatter_params(), tm, tk, bpa, nt), "can't find a sutable scatter");
+  gs_check(out, in, idx, bpa);
+  const int64_t m = idx.shape[0];
+  CS_ARG(in.shape[0] >= m, "input has ", in.shape[0], " rows, indices ", m);
+  CS_HIP(cs::launch_scatter_add_f32(static_cast<const float*>(in.data), static_cast<float*>(out.data),
+                                    reinterpret_cast<const int32_t*>(idx.data), m, out.shape[0], in.shape[1],
+                                    row_stride(in), row_stride(out), reinterpret_cast<hipStream_t>(stream)));
+}
+
+int copy_params(const std::vector<GsParam>& v, int32_t* out, int cap) {
+  if (out != nullptr)
+    for (int i = 0; i < cap && i < (int)v.size(); ++i)
+      for (int j = 0; j < 4; ++j) out[4 * i + j] = v[i][j];
+  return (int)v.size();
+}
+
 }  // namespace
 
 // ====================================================================== C ABI
@@ -1014,5 +1155,35 @@ int cs_stream_synchronize(uint64_t stream) {
 }
 
 const char* cs_last_error(void) { return g_last_error.c_str(); }
+
+int cs_get_all_gather_params(int32_t* out, int cap) { return copy_params(gather_params(), out, cap); }
+
+int cs_get_all_scatter_params(int32_t* out, int cap) { return copy_params(scatter_params(), out, cap); }
+
+int cs_gather_supported(int32_t bytes_per_access, int32_t channel_size, int32_t dtype) {
+  return gs_supported(bytes_per_access, channel_size, dtype) ? 1 : 0;
+}
+
+int cs_scatter_supported(int64_t tile_m, int64_t tile_k_bytes, int64_t bytes_per_access, int64_t num_threads,
+                         int32_t channel_size, int32_t dtype) {
+  if (dtype != 0 || !has_param(scatter_params(), tile_m, tile_k_bytes, bytes_per_access, num_threads)) return 0;
+  return gs_supported(bytes_per_access, channel_size, dtype) ? 1 : 0;
+}
+
+int cs_gather(const cs_tensor* output, const cs_tensor* input, const cs_tensor* indices, int64_t tile_m,
+              int64_t tile_k_bytes, int64_t bytes_per_access, int64_t num_threads, uint64_t stream) {
+  return guarded([&] {
+    CS_ARG(output != nullptr && input != nullptr && indices != nullptr, "null argument");
+    run_gather(*output, *input, *indices, tile_m, tile_k_bytes, bytes_per_access, num_threads, stream);
+  });
+}
+
+int cs_scatter(const cs_tensor* output, const cs_tensor* input, const cs_tensor* indices, int64_t tile_m,
+               int64_t tile_k_bytes, int64_t bytes_per_access, int64_t num_threads, uint64_t stream) {
+  return guarded([&] {
+    CS_ARG(output != nullptr && input != nullptr && indices != nullptr, "null argument");
+    run_scatter(*output, *input, *indices, tile_m, tile_k_bytes, bytes_per_access, num_threads, stream);
+  });
+}
 
 }  // extern "C"

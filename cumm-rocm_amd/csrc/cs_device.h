@@ -11,6 +11,8 @@
 //   f16  : 32 elements -> 1 x v_mfma_f32_16x16x32_f16
 //   bf16 : 32 elements -> 1 x v_mfma_f32_16x16x32_bf16
 //   e4m3 : 64 elements -> 2 x v_mfma_f32_16x16x32_fp8_fp8
+//   int8 : 64 elements -> 1 x v_mfma_i32_16x16x64_i8 (int32 accumulators,
+//          kept as bit patterns in the same f32x4 registers)
 // The 16x16 accumulator holds, in lane l, column (l & 15) and rows
 // 4*(l >> 4) + i, i = 0..3.
 #pragma once
@@ -23,6 +25,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
 
 struct bf16 { uint16_t bits; };
 struct e4m3 { uint8_t bits; };
@@ -50,6 +53,7 @@ __device__ __forceinline__ float to_f32(_Float16 v) { return (float)v; }
 __device__ __forceinline__ float to_f32(bf16 v) {
   return __uint_as_float(((uint32_t)v.bits) << 16);
 }
+__device__ __forceinline__ float to_f32(int8_t v) { return (float)v; }
 
 template <typename OutT> __device__ __forceinline__ OutT from_f32(float v);
 template <> __device__ __forceinline__ float from_f32<float>(float v) { return v; }
@@ -67,6 +71,11 @@ template <> __device__ __forceinline__ bf16 from_f32<bf16>(float v) {
     r.bits = (uint16_t)(u >> 16);
   }
   return r;
+}
+template <> __device__ __forceinline__ int8_t from_f32<int8_t>(float v) {
+  // clamp to [-128, 127], then round to nearest even (NaN -> -128)
+  v = fminf(fmaxf(v, -128.f), 127.f);
+  return (int8_t)(int)__builtin_rintf(v);
 }
 
 // ---------------------------------------------------------------- MFMA step
@@ -112,6 +121,19 @@ template <> struct Mma<e4m3> {
   }
 };
 
+template <> struct Mma<int8_t> {
+  static constexpr int kElems = 16;
+  static constexpr int kChunk = 64;
+  __device__ __forceinline__ static void step(f32x4& acc, u32x4 a, u32x4 b) {
+    const i32x4 r = __builtin_amdgcn_mfma_i32_16x16x64_i8(__builtin_bit_cast(i32x4, a), __builtin_bit_cast(i32x4, b),
+                                                          __builtin_bit_cast(i32x4, acc), 0, 0, 0);
+    acc = __builtin_bit_cast(f32x4, r);
+  }
+};
+
+template <typename T> struct IsInt8 { static constexpr bool value = false; };
+template <> struct IsInt8<int8_t> { static constexpr bool value = true; };
+
 // ---------------------------------------------------------------- epilogue
 // D = act(alpha * acc + beta * source); act codes follow tv::gemm::Activation.
 enum : int { kActNone = 0, kActReLU = 1, kActSigmoid = 2, kActLeakyReLU = 3 };
@@ -123,6 +145,18 @@ __device__ __forceinline__ float apply_act(float v, int act, float act_alpha) {
     case kActLeakyReLU: return v >= 0.f ? v : v * act_alpha;
     default: return v;
   }
+}
+
+// int8 inference (Int8Inference output op, cumm/gemm/output_op/linear.py:198-392,
+// bias not pre-scaled): v = act(scale[n] * acc + bias[n] + beta * source[o][n]),
+// fused multiply-adds in float; source is output_add (or the output) row o.
+// The int8 conversion (clamp, round to nearest even) is from_f32<int8_t>.
+// sc / bi are scale[n] / bias[n], which the kernels keep in registers.
+template <typename Args>
+__device__ __forceinline__ float i8_epilogue(const Args& p, int32_t acc, float sc, float bi, int n, int64_t o) {
+  float v = __builtin_fmaf(sc, (float)acc, bi);
+  if (p.beta != 0.f) v = __builtin_fmaf(p.beta, to_f32(reinterpret_cast<const int8_t*>(p.d)[o * p.ldd + n]), v);
+  return apply_act(v, p.act_type, p.act_alpha);
 }
 
 }  // namespace cs

@@ -7,7 +7,7 @@ namespace cs {
 
 // Element type tags used by the launchers; values are tv::DType codes.
 enum DT : int {
-  kF32 = 0, kI32 = 1, kF64 = 4, kF16 = 7, kU32 = 10, kBF16 = 12, kE4M3 = 14
+  kF32 = 0, kI32 = 1, kI8 = 3, kF64 = 4, kF16 = 7, kU32 = 10, kBF16 = 12, kE4M3 = 14
 };
 
 // Row-gather implicit GEMM (sparse conv forward / dgrad, gather-A scatter-C GEMM).
@@ -34,6 +34,8 @@ struct RowGatherArgs {
   const uint32_t* mask;      // sorted per-row masks; null -> every offset of the filter
   uint32_t* mask_out;        // per-16-row tile masks (forward), may be null
   const float* scale;        // per-output-column multiplier, may be null
+  // int8 inference (int8 a, b, c): c = act(scale[n] * acc + bias_f[n] + beta * d[o][n])
+  const float* bias_f;
   int32_t m, kred, nout, kv;
   uint32_t mask_filter;
   int32_t reverse_mask;      // dgrad of a submanifold layer reuses forward masks
@@ -93,6 +95,7 @@ hipError_t launch_rowgather_f32(int out_dtype, int block_n, const RowGatherArgs&
 hipError_t launch_rowgather_f16(int out_dtype, int block_n, const RowGatherArgs& args, hipStream_t s);
 hipError_t launch_rowgather_bf16(int out_dtype, int block_n, const RowGatherArgs& args, hipStream_t s);
 hipError_t launch_rowgather_e4m3(int out_dtype, int block_n, const RowGatherArgs& args, hipStream_t s);
+hipError_t launch_rowgather_i8(int out_dtype, int block_n, const RowGatherArgs& args, hipStream_t s);
 hipError_t launch_rowgather(int in_dtype, int out_dtype, int block_n, const RowGatherArgs& args, hipStream_t s);
 bool rowgather_supported(int in_dtype, int out_dtype);
 int rowgather_block_n(int nout);
@@ -104,6 +107,12 @@ hipError_t launch_transpose(const void* in, void* out, int64_t R, int64_t F, int
                             int64_t in_row_stride, int elem_bytes, hipStream_t s);
 // 2-D copy with strides (elements), used to make operands dense.
 hipError_t launch_zero(void* p, int64_t nbytes, hipStream_t s);
+// out[i] = in[idx[i]] for m rows of row_bytes bytes (idx outside [0, n_in): zeros)
+hipError_t launch_gather_rows(const void* in, void* out, const int32_t* idx, int64_t m, int64_t n_in,
+                              int64_t row_bytes, int64_t in_ld_bytes, int64_t out_ld_bytes, hipStream_t s);
+// out[idx[i]] += in[i] for m float rows of `cols` (idx outside [0, n_out): skipped)
+hipError_t launch_scatter_add_f32(const float* in, float* out, const int32_t* idx, int64_t m, int64_t n_out,
+                                  int64_t cols, int64_t in_ld, int64_t out_ld, hipStream_t s);
 
 constexpr int kRowsPerWave = 64;     // four 16-row mask tiles per wave
 constexpr int kWavesPerBlock = 4;

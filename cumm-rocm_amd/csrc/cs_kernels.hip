@@ -426,6 +426,7 @@ bool rowgather_supported(int in_dtype, int out_dtype) {
     case kF16: return out_dtype == kF16 || out_dtype == kF32;
     case kBF16: return out_dtype == kBF16 || out_dtype == kF32;
     case kE4M3: return out_dtype == kF16 || out_dtype == kBF16 || out_dtype == kF32;
+    case kI8: return out_dtype == kI8;
     default: return false;
   }
 }
@@ -436,6 +437,7 @@ hipError_t launch_rowgather(int in_dtype, int out_dtype, int block_n, const RowG
     case kF16: return launch_rowgather_f16(out_dtype, block_n, a, s);
     case kBF16: return launch_rowgather_bf16(out_dtype, block_n, a, s);
     case kE4M3: return launch_rowgather_e4m3(out_dtype, block_n, a, s);
+    case kI8: return launch_rowgather_i8(out_dtype, block_n, a, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -206,17 +206,25 @@ __global__ __launch_bounds__(256) void rowgather_kernel(RowGatherArgs p) {
   const int n = n0 + lane;
   const bool ncol = n < p.nout;
   const float scl = (p.scale && ncol) ? p.scale[n] : 1.f;
+  const float bia = (IsInt8<T>::value && ncol) ? p.bias_f[n] : 0.f;
   const float bias = (p.beta != 0.f && p.d_is_bias && ncol) ? to_f32(D[n]) : 0.f;
 #pragma unroll 4
   for (int i = 0; i < 16; ++i) {
     const int rr = 16 * s + i;
     const int orow = __shfl(myout, rr);
     if (row0 + rr < p.m && ncol) {
-      float v = ((red[0][rr][lane] + red[1][rr][lane]) + red[2][rr][lane]) + red[3][rr][lane];
-      v *= scl;
-      v *= p.alpha;
-      if (p.beta != 0.f) v += p.beta * (p.d_is_bias ? bias : to_f32(D[(int64_t)orow * p.ldd + n]));
-      v = apply_act(v, p.act_type, p.act_alpha);
+      float v;
+      if constexpr (IsInt8<T>::value) {
+        const int32_t acc = ((__float_as_int(red[0][rr][lane]) + __float_as_int(red[1][rr][lane])) +
+                             __float_as_int(red[2][rr][lane])) + __float_as_int(red[3][rr][lane]);
+        v = i8_epilogue(p, acc, scl, bia, n, orow);
+      } else {
+        v = ((red[0][rr][lane] + red[1][rr][lane]) + red[2][rr][lane]) + red[3][rr][lane];
+        v *= scl;
+        v *= p.alpha;
+        if (p.beta != 0.f) v += p.beta * (p.d_is_bias ? bias : to_f32(D[(int64_t)orow * p.ldd + n]));
+        v = apply_act(v, p.act_type, p.act_alpha);
+      }
       C[(int64_t)orow * p.ldc + n] = from_f32<OutT>(v);
     }
   }

@@ -173,6 +173,16 @@ __global__ __launch_bounds__(64 * kStWaves) void rowgather_stream_kernel(RowGath
   // ---- epilogue: accumulator element i of column block nb is row 4*lg + i,
   // column nb*16 + lr
   if (!live) return;
+  // per-column constants of this lane's columns, loaded together after the
+  // main loop (one round trip, no registers held across it)
+  float ep_s[NB], ep_b[NB];
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) {
+    const int n = nb * 16 + lr;
+    const bool ok = n < p.nout;
+    ep_s[nb] = (p.scale && ok) ? p.scale[n] : 1.f;
+    ep_b[nb] = (IsInt8<T>::value && ok) ? p.bias_f[n] : 0.f;
+  }
   OutT* Cout = reinterpret_cast<OutT*>(p.c);
   const OutT* D = reinterpret_cast<const OutT*>(p.d);
 #pragma unroll
@@ -184,14 +194,19 @@ __global__ __launch_bounds__(64 * kStWaves) void rowgather_stream_kernel(RowGath
       for (int nb = 0; nb < NB; ++nb) {
         const int n = nb * 16 + lr;
         if (n < p.nout) {
-          float v = acc[nb][i];
-          if (p.scale) v *= p.scale[n];
-          v *= p.alpha;
-          if (p.beta != 0.f) {
-            const float src = p.d_is_bias ? to_f32(D[n]) : to_f32(D[(int64_t)o * p.ldd + n]);
-            v += p.beta * src;
+          float v;
+          if constexpr (IsInt8<T>::value) {
+            v = i8_epilogue(p, __float_as_int(acc[nb][i]), ep_s[nb], ep_b[nb], n, o);
+          } else {
+            v = acc[nb][i];
+            if (p.scale) v *= ep_s[nb];
+            v *= p.alpha;
+            if (p.beta != 0.f) {
+              const float src = p.d_is_bias ? to_f32(D[n]) : to_f32(D[(int64_t)o * p.ldd + n]);
+              v += p.beta * src;
+            }
+            v = apply_act(v, p.act_type, p.act_alpha);
           }
-          v = apply_act(v, p.act_type, p.act_alpha);
           Cout[(int64_t)o * p.ldc + n] = from_f32<OutT>(v);
         }
       }

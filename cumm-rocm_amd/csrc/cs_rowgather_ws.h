@@ -146,6 +146,16 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
   if (tid == 0) tile_ctr = 2 * kWsWaves;
   __syncthreads();
 
+  // per-column epilogue constants of this lane's columns, loaded once
+  float ep_s[NBC], ep_b[NBC];
+#pragma unroll
+  for (int nb = 0; nb < NBC; ++nb) {
+    const int n = n0 + nb * 16 + lr;
+    const bool ok = n < p.nout;
+    ep_s[nb] = (p.scale && ok) ? p.scale[n] : 1.f;
+    ep_b[nb] = (IsInt8<T>::value && ok) ? p.bias_f[n] : 0.f;
+  }
+
   const __amdgpu_buffer_rsrc_t ra = make_rsrc(p.a, p.a_bytes);
   OutT* Cout = reinterpret_cast<OutT*>(p.c);
   const OutT* D = reinterpret_cast<const OutT*>(p.d);
@@ -249,9 +259,14 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
 #pragma unroll
         for (int nb = 0; nb < NBC; ++nb) {
           const int n = n0 + nb * 16 + lr;
-          float v = acc[nb][i];
-          if (p.scale) v *= p.scale[n];
-          v = apply_act(v * p.alpha, p.act_type, p.act_alpha);
+          float v;
+          if constexpr (IsInt8<T>::value) {
+            v = i8_epilogue(p, __float_as_int(acc[nb][i]), ep_s[nb], ep_b[nb], n, 0);  // beta == 0 here
+          } else {
+            v = acc[nb][i];
+            if (p.scale) v *= ep_s[nb];
+            v = apply_act(v * p.alpha, p.act_type, p.act_alpha);
+          }
           st[(4 * lg + i) * NCOL + nb * 16 + lr] = from_f32<OutT>(v);
         }
       __builtin_amdgcn_wave_barrier();
@@ -280,14 +295,19 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
         for (int nb = 0; nb < NBC; ++nb) {
           const int n = n0 + nb * 16 + lr;
           if (n < p.nout) {
-            float v = acc[nb][i];
-            if (p.scale) v *= p.scale[n];
-            v *= p.alpha;
-            if (p.beta != 0.f) {
-              const float src = p.d_is_bias ? to_f32(D[n]) : to_f32(D[(int64_t)o * p.ldd + n]);
-              v += p.beta * src;
+            float v;
+            if constexpr (IsInt8<T>::value) {
+              v = i8_epilogue(p, __float_as_int(acc[nb][i]), ep_s[nb], ep_b[nb], n, o);
+            } else {
+              v = acc[nb][i];
+              if (p.scale) v *= ep_s[nb];
+              v *= p.alpha;
+              if (p.beta != 0.f) {
+                const float src = p.d_is_bias ? to_f32(D[n]) : to_f32(D[(int64_t)o * p.ldd + n]);
+                v += p.beta * src;
+              }
+              v = apply_act(v, p.act_type, p.act_alpha);
             }
-            v = apply_act(v, p.act_type, p.act_alpha);
             Cout[(int64_t)o * p.ldc + n] = from_f32<OutT>(v);
           }
         }

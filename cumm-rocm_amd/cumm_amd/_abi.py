@@ -29,6 +29,8 @@ SYMBOLS = [
     "cs_stream_synchronize", "cs_last_error",
     "cs_pairs_subm_workspace", "cs_pairs_subm", "cs_pairs_strided_out_workspace", "cs_pairs_strided_out",
     "cs_pairs_strided_workspace", "cs_pairs_strided", "cs_pairs_last_error",
+    "cs_get_all_gather_params", "cs_get_all_scatter_params", "cs_gather_supported", "cs_scatter_supported",
+    "cs_gather", "cs_scatter",
 ]
 
 
@@ -136,6 +138,16 @@ def lib() -> C.CDLL:
                  "cs_pairs_strided_out", "cs_pairs_strided_workspace", "cs_pairs_strided"):
         getattr(L, name).restype = C.c_int
     L.cs_pairs_last_error.restype = C.c_char_p
+    L.cs_get_all_gather_params.argtypes = [P, C.c_int]
+    L.cs_get_all_scatter_params.argtypes = [P, C.c_int]
+    L.cs_gather_supported.argtypes = [I32, I32, I32]
+    L.cs_scatter_supported.argtypes = [I64, I64, I64, I64, I32, I32]
+    T = C.POINTER(CsTensor)
+    L.cs_gather.argtypes = [T, T, T, I64, I64, I64, I64, U64]
+    L.cs_scatter.argtypes = [T, T, T, I64, I64, I64, I64, U64]
+    for name in ("cs_get_all_gather_params", "cs_get_all_scatter_params", "cs_gather_supported",
+                 "cs_scatter_supported", "cs_gather", "cs_scatter"):
+        getattr(L, name).restype = C.c_int
     if L.cs_abi_version() != ABI_VERSION:
         raise RuntimeError("libcumm_amd.so ABI version mismatch")
     _lib = L
@@ -153,10 +165,27 @@ def check(rc: int):
 
 # -------------------------------------------------------------- conversions
 def to_cs_tensor(t: tv.Tensor) -> CsTensor:
-    ct = CsTensor()
+    """The borrowed view as a cs_tensor.  The struct is cached on the
+    tv.Tensor and rebuilt whenever the torch view's pointer, shape or strides
+    differ from the cached ones (a torch tensor's dtype and device never
+    change in place).  Callers copy the returned struct and never modify it."""
     if t is None or t.empty():
-        return ct
+        return _EMPTY
     tt = t.torch()
+    key = (tt.data_ptr(), tt.shape, tt.stride())
+    hit = t._abi
+    if hit is not None and hit[0] == key:
+        return hit[1]
+    ct = _make_cs_tensor(t, tt)
+    t._abi = (key, ct)
+    return ct
+
+
+_EMPTY = CsTensor()
+
+
+def _make_cs_tensor(t: tv.Tensor, tt) -> CsTensor:
+    ct = CsTensor()
     if tt.dim() > MAX_DIM:
         raise ValueError(f"rank {tt.dim()} > {MAX_DIM}")
     ct.data = tt.data_ptr() if tt.numel() > 0 else None
@@ -173,7 +202,23 @@ def to_cs_tensor(t: tv.Tensor) -> CsTensor:
     return ct
 
 
+def _desp_lists(d):
+    # list fields can change in place without passing through __setattr__
+    return (tuple(d.tile_shape), tuple(d.warp_tile_shape), tuple(d.tensorop), tuple(d.min_arch))
+
+
 def desp_to_cs(d) -> CsGemmAlgoDesp:
+    """GemmAlgoDesp -> struct, cached on the desp until a field changes."""
+    key = _desp_lists(d)
+    hit = getattr(d, "_abi_gemm", None)
+    if hit is not None and hit[0] == key:
+        return hit[1]
+    c = _make_desp(d)
+    object.__setattr__(d, "_abi_gemm", (key, c))
+    return c
+
+
+def _make_desp(d) -> CsGemmAlgoDesp:
     c = CsGemmAlgoDesp()
     c.dtype_a, c.dtype_b, c.dtype_c = int(d.dtype_a), int(d.dtype_b), int(d.dtype_c)
     c.trans_a, c.trans_b, c.trans_c = int(d.trans_a_), int(d.trans_b_), int(d.trans_c_)
@@ -196,8 +241,19 @@ def desp_to_cs(d) -> CsGemmAlgoDesp:
 
 
 def conv_desp_to_cs(d) -> CsConvAlgoDesp:
+    """ConvAlgoDesp -> struct, cached on the desp until a field changes."""
+    key = _desp_lists(d)
+    hit = getattr(d, "_abi_conv", None)
+    if hit is not None and hit[0] == key:
+        return hit[1]
+    c = _make_conv_desp(d)
+    object.__setattr__(d, "_abi_conv", (key, c))
+    return c
+
+
+def _make_conv_desp(d) -> CsConvAlgoDesp:
     c = CsConvAlgoDesp()
-    c.gemm = desp_to_cs(d)
+    c.gemm = _make_desp(d)
     c.ndim, c.op_type, c.iter_algo = int(d.ndim), int(d.op_type), int(d.iter_algo)
     c.layout_i, c.layout_w, c.layout_o = int(d.layout_i), int(d.layout_w), int(d.layout_o)
     c.interleave_i, c.interleave_w, c.interleave_o = int(d.interleave_i), int(d.interleave_w), int(d.interleave_o)

@@ -98,10 +98,11 @@ class Tensor:
     parameters such as ``workspace`` or ``bias``.
     """
 
-    __slots__ = ("_t",)
+    __slots__ = ("_t", "_abi")
 
     def __init__(self, t: Optional[torch.Tensor] = None):
         self._t = t
+        self._abi = None  # (view key, C-ABI struct) cached by the binding
 
     # -- construction ------------------------------------------------------
     @staticmethod

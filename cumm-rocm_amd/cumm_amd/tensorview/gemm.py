@@ -70,6 +70,12 @@ class NVRTCParams:
 class GemmAlgoDesp:
     """tv::gemm::GemmAlgoDesp (params.h:24-164)."""
 
+    def __setattr__(self, name, value):
+        # a field change drops the C-ABI structs the binding cached for this desp
+        object.__setattr__(self, "_abi_gemm", None)
+        object.__setattr__(self, "_abi_conv", None)
+        object.__setattr__(self, name, value)
+
     def __init__(self):
         self.dtype_a = unknown
         self.dtype_b = unknown

@@ -179,6 +179,34 @@ int cs_pairs_strided(const int32_t* in_coords, int64_t n_in, const int32_t* in_s
                      int64_t ws_bytes, uint64_t stream);
 const char* cs_pairs_last_error(void);
 
+/* ---- Standalone row gather / scatter-add, the glue around matmul2 of
+ * spconv's gather-GEMM-scatter ("native") algorithm.
+ *   GatherAll.get_all_gather_params / ScatterAll.get_all_scatter_params
+ *     (cumm/gemm/gather.py:445-455, 552-570): writes up to `cap` entries of
+ *     (tile_m, tile_k_bytes, bytes_per_access, num_threads) into out[4*i..],
+ *     returns the total.
+ *   GatherAll.supported (gather.py:459-468): (channel_size * itemsize) %
+ *     bytes_per_access == 0.  ScatterAll.supported_scatter (gather.py:574-589)
+ *     additionally needs a published float32 parameter set.
+ *   GatherAll.gather (gather.py:482-495 -> Gather.gather 224-257):
+ *     output[i] = input[indices[i]] for i < indices.dim(0); any dtype.
+ *   ScatterAll.scatter (gather.py:603-618 -> Scatter.scatter 356-389):
+ *     output[indices[i]] += input[i]; float32; the indices of one call must be
+ *     distinct (the reference adds without atomics).
+ * Tile parameters must be a published set ("can't find a sutable gather" /
+ * "...scatter" otherwise, code 2); the kernel's own launch shape is chosen for
+ * the row width.  Rows whose index lies outside the source (gather: reads
+ * zeros) or target (scatter: skipped) are not read or written. */
+int cs_get_all_gather_params(int32_t* out, int cap);
+int cs_get_all_scatter_params(int32_t* out, int cap);
+int cs_gather_supported(int32_t bytes_per_access, int32_t channel_size, int32_t dtype);
+int cs_scatter_supported(int64_t tile_m, int64_t tile_k_bytes, int64_t bytes_per_access, int64_t num_threads,
+                         int32_t channel_size, int32_t dtype);
+int cs_gather(const cs_tensor* output, const cs_tensor* input, const cs_tensor* indices, int64_t tile_m,
+              int64_t tile_k_bytes, int64_t bytes_per_access, int64_t num_threads, uint64_t stream);
+int cs_scatter(const cs_tensor* output, const cs_tensor* input, const cs_tensor* indices, int64_t tile_m,
+               int64_t tile_k_bytes, int64_t bytes_per_access, int64_t num_threads, uint64_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,6 +18,11 @@ Pinning (tests/test_oracle.py):
                          same pairs, cpu_dgrad, reverse-mask equivalence for subm.
   * conv_wgrad        -- adjoint identity <fwd(X;W), dO> == <W, wgrad(X, dO)>,
                          cpu_wgrad, reduced-mask skipping on hand-built masks.
+  * int8_inference_epilogue -- hand-computed known answers for rounding ties
+                         (half to even), saturation at -128 / 127, bias and
+                         output_add terms, activation before clamping; with
+                         scale 1 and bias 0 the int8 conv equals the integer
+                         brute-force sum clipped to [-128, 127].
 """
 from __future__ import annotations
 
@@ -48,6 +53,35 @@ def epilogue(acc: np.ndarray, alpha: float, beta: float, src: Optional[np.ndarra
     if beta != 0.0 and src is not None:
         v = v + beta * src
     return _act(v, act_type, act_alpha)
+
+
+def int8_inference_epilogue(acc, scale, bias, beta: float = 0.0, source=None, act_type: int = 0,
+                            act_alpha: float = 0.0) -> np.ndarray:
+    """Int8Inference output op without pre-scaled bias
+    (cumm/gemm/output_op/linear.py:198-392, used when ``is_int8_inference``,
+    cumm/gemm/algospec/turing.py:480-500): in float32,
+    ``v = scale * float(acc) + bias`` (one fused multiply-add), then
+    ``v = beta * source + v`` when beta != 0, then the activation, then clamp to
+    [-128, 127] and round to nearest, ties to even, into int8.  alpha is not
+    used.  acc is the exact int32 accumulator [N, K]; scale / bias are per
+    output channel; source is output_add (or the output) [N, K] int8.
+
+    Each fused multiply-add is evaluated exactly in float64 (the product of two
+    float32 values is exact there) and rounded once to float32; this equals a
+    float32 FMA unless the float64 sum rounds onto a float32 half-way point,
+    which needs a 29-bit coincidence and does not occur in the tests' data.
+    """
+    f32 = np.float32
+    a = np.asarray(acc, np.int64).astype(f32).astype(np.float64)
+    sc = np.asarray(scale, f32).astype(np.float64)[None, :a.shape[1]]
+    bi = np.asarray(bias, f32).astype(np.float64)[None, :a.shape[1]]
+    v = (sc * a + bi).astype(f32)
+    if beta != 0.0:
+        src = np.asarray(source, np.int64).astype(np.float64)
+        v = (np.float64(f32(beta)) * src + v.astype(np.float64)).astype(f32)
+    v = _act(v.astype(np.float64), act_type, float(f32(act_alpha))).astype(f32)
+    v = np.clip(v, f32(-128), f32(127))
+    return np.rint(v).astype(np.int8)
 
 
 def tile_masks(mask_sorted: np.ndarray, n: int, tile_m: int, mask_filter: int = ALL) -> np.ndarray:

@@ -163,7 +163,7 @@ def test_catalogue_selection():
     w = select_conv_algo(2, tv.float16, tv.float32, tv.float16, 64, 64)
     assert w.split_k_parallel and w.dtype_input == tv.float16 and w.dtype_weight == tv.float32
     with pytest.raises(ValueError):
-        select_conv_algo(0, tv.int8, tv.int8, tv.int8, 64, 64)
+        select_conv_algo(1, tv.int8, tv.int8, tv.int8, 64, 64)  # int8 is forward (inference) only
 
 
 # --------------------------------------------------------------- validation (host only)
@@ -247,3 +247,86 @@ def test_gemm_validation_errors():
     g.b = tv.from_torch(torch.zeros((12, 4)) if not d.trans_b else torch.zeros((4, 12)))
     with pytest.raises(ValueError):
         GemmMainUnitTest.matmul2(g)  # k != k2
+
+
+# --------------------------------------------------------------- gather / scatter
+def test_gather_scatter_params_and_supported():
+    from cumm_amd.gemm.gather import GatherAll, ScatterAll
+    g = GatherAll.get_all_gather_params()
+    s = ScatterAll.get_all_scatter_params()
+    # a set is published iff num_threads * bytes_per_access <= tile_m * tile_k_bytes
+    # (cumm/gemm/gather.py:407, 516); the smallest tile that holds 64 threads x 4 B
+    # is 8 x 32 B, the first entry of the reference's tile-k-major enumeration
+    assert g[0] == (8, 32, 4, 64)
+    assert (1, 256, 4, 64) in g and (1, 256, 8, 64) not in g and (8, 2048, 16, 256) in g
+    assert (1, 4, 4, 64) not in g
+    assert len(set(g)) == len(g) and len(set(s)) == len(s)
+    assert all(nt * b <= tm * tk for tm, tk, b, nt in g + s)
+    assert all(nt in (128, 256) and tk >= 64 for _, tk, _, nt in s)
+    assert s[0] == (8, 64, 4, 128) and (4, 64, 4, 128) not in s
+    # supported: (channel_size * itemsize) % bytes_per_access == 0
+    assert GatherAll.supported(16, 4, tv.float32) and not GatherAll.supported(16, 6, tv.float32)
+    assert GatherAll.supported(8, 4, tv.float16) and not GatherAll.supported(16, 4, tv.float16)
+    assert GatherAll.supported(4, 12, tv.int8) and not GatherAll.supported(8, 12, tv.int8)
+    sa = ScatterAll()
+    assert sa.supported_scatter(8, 64, 4, 128, 3, tv.float32)
+    assert not sa.supported_scatter(8, 64, 4, 128, 3, tv.float16)   # float32 kernels only
+    assert not sa.supported_scatter(8, 64, 4, 64, 4, tv.float32)    # unpublished set
+    assert not sa.supported_scatter(8, 256, 16, 128, 2, tv.float32)  # 8 B rows vs 16 B access
+
+
+def test_gather_scatter_validation():
+    from cumm_amd.gemm.gather import GatherAll, ScatterAll
+    x = tv.from_torch(torch.zeros((10, 8), dtype=torch.float32))
+    y = tv.from_torch(torch.zeros((4, 8), dtype=torch.float32))
+    idx = tv.from_torch(torch.zeros((4,), dtype=torch.int32))
+    with pytest.raises(RuntimeError, match="sutable gather"):
+        GatherAll().gather(y, x, idx, 1, 4, 4, 64)
+    with pytest.raises(RuntimeError, match="sutable scatter"):
+        ScatterAll().scatter(x, y, idx, 8, 64, 4, 64)
+    with pytest.raises(RuntimeError, match="not supported"):
+        GatherAll().gather(tv.from_torch(torch.zeros((4, 3), dtype=torch.float16)),
+                           tv.from_torch(torch.zeros((10, 3), dtype=torch.float16)), idx, 8, 32, 4, 64)
+    with pytest.raises(RuntimeError):  # channel mismatch
+        GatherAll().gather(tv.from_torch(torch.zeros((4, 4), dtype=torch.float32)), x, idx, 8, 32, 4, 64)
+    with pytest.raises(ValueError, match="GPU"):
+        GatherAll().gather(y, x, idx, 8, 32, 4, 64)  # host tensors never reach a kernel
+
+
+def test_int8_inference_catalogue_and_validation():
+    run = ConvMainUnitTest.implicit_gemm2
+    i8 = [d for d in ConvMainUnitTest.get_all_conv_algo_desp() if d.is_int8_inference]
+    assert sorted(d.tile_shape[1] for d in i8) == [16, 32, 64, 128]
+    for d in i8:
+        assert int(d.op_type) == 0 and (d.dtype_a, d.dtype_b, d.dtype_c) == (tv.int8, tv.int8, tv.int8)
+        assert d.dacc == tv.int32 and d.dcomp == tv.float32
+        assert repr(d).endswith("_S8") and d.element_per_access_a == 16
+    d = select_conv_algo(0, tv.int8, tv.int8, tv.int8, 64, 64)
+    assert d.is_int8_inference and d.tile_shape[1] == 64
+
+    def params(**kw):
+        N, C, K = 100, 64, 32
+        p = ConvParams(3, ConvOpType.Forward)
+        p.conv_algo_desp = select_conv_algo(0, tv.int8, tv.int8, tv.int8, C, K)
+        p.input = tv.from_torch(torch.zeros((N, C), dtype=torch.int8))
+        p.weight = tv.from_torch(torch.zeros((K, 27, C), dtype=torch.int8))
+        p.output = tv.from_torch(torch.zeros((N, K), dtype=torch.int8))
+        p.indices = tv.from_torch(torch.full((27, N), -1, dtype=torch.int32))
+        p.mask = tv.from_torch(torch.zeros(N, dtype=torch.int32))
+        p.mask_argsort = tv.from_torch(torch.arange(N, dtype=torch.int32))
+        p.bias = tv.from_torch(kw.get("bias", torch.zeros(K)))
+        p.scale = tv.from_torch(torch.ones(K))
+        if "output_add" in kw:
+            p.output_add = tv.from_torch(kw["output_add"])
+        return p
+
+    p = params()
+    p.scale = tv.Tensor()
+    with pytest.raises(RuntimeError, match="scale and bias"):
+        run(p)  # nvrtc_code.py:81-82
+    with pytest.raises(RuntimeError, match="same dtype and shape"):
+        run(params(output_add=torch.zeros((100, 16), dtype=torch.int8)))  # nvrtc_code.py:85-87
+    with pytest.raises(RuntimeError, match="same dtype and shape"):
+        run(params(output_add=torch.zeros((100, 32), dtype=torch.float16)))
+    with pytest.raises(ValueError, match="GPU"):
+        run(params(output_add=torch.zeros((100, 32), dtype=torch.int8)))  # otherwise valid, host tensors

@@ -151,6 +151,41 @@ def test_forward_epilogue():
     np.testing.assert_allclose(got, 1 / (1 + np.exp(-plain)), rtol=1e-12)
 
 
+def test_int8_inference_epilogue_known_answers():
+    e = ref.int8_inference_epilogue
+    one, zero = np.ones(1), np.zeros(1)
+    col = lambda v: np.array(v, np.int64).reshape(-1, 1)
+    # round half to even after scale * acc + bias
+    got = e(col([0, 1, 2, 3, -1, -2, -3]), one, [0.5])
+    assert got.ravel().tolist() == [0, 2, 2, 4, 0, -2, -2]
+    # saturation at both ends, exact at the limits
+    assert e(col([127, 128, 1000, -128, -129, -5000]), one, zero).ravel().tolist() == [127, 127, 127, -128, -128, -128]
+    # scale then bias: 0.25 * 10 + 1.0 = 3.5 -> 4 ; 0.25 * 6 - 1 = 0.5 -> 0
+    assert e(col([10, 6]), [0.25], [1.0]).ravel().tolist() == [4, 2]
+    assert e(col([6]), [0.25], [-1.0]).ravel().tolist() == [0]
+    # output_add with beta: 2 + 0.5 * 7 = 5.5 -> 6 ; alpha plays no part
+    assert e(col([2]), one, zero, beta=0.5, source=col([7])).ravel().tolist() == [6]
+    # activation before clamping: ReLU(-300) = 0 ; LeakyReLU(-300, 0.1) = -30
+    assert e(col([-300]), one, zero, act_type=1).ravel().tolist() == [0]
+    assert e(col([-300]), one, zero, act_type=3, act_alpha=0.1).ravel().tolist() == [-30]
+    # per-column scale and bias are indexed by output channel
+    acc = np.array([[4, 4, 4]], np.int64)
+    assert e(acc, [1.0, 2.0, 0.5], [0.0, 1.0, -1.0]).ravel().tolist() == [4, 9, 1]
+
+
+def test_int8_conv_equals_clipped_integer_bruteforce():
+    prob, _, _, _ = _tiny(seed=12, C=16, K=8)
+    rng = np.random.default_rng(3)
+    X = rng.integers(-12, 13, (prob.n, 16))
+    W = rng.integers(-6, 7, (8, 27, 16))
+    acc, _ = ref.conv_forward(X, W, prob.pair_fwd, prob.mask_sorted, prob.argsort, 16)
+    bf = bruteforce.conv_forward_coords(prob.coords, X, W)
+    assert np.array_equal(acc, bf)
+    got = ref.int8_inference_epilogue(acc.astype(np.int64), np.ones(8), np.zeros(8))
+    assert np.array_equal(got, np.clip(bf, -128, 127).astype(np.int8))
+    assert (np.abs(bf) > 127).any() and (np.abs(bf) < 127).any()
+
+
 # --------------------------------------------------------------- dgrad / wgrad
 def test_dgrad_adjoint_and_cpu_path():
     prob, X, W, dO = _tiny(seed=8)
