@@ -368,6 +368,23 @@ const std::vector<cs_conv_algo_desp>& conv_catalogue() {
       // wgrad: a = dout, b = input, c = dW; trans (T, F, F)
       v.push_back(conv_desp(2, wgrad_desp(c.in, c.w, c.out, cs::kMaskTile)));
     }
+    // dynamic (multi-word) masks for kv > 32: forward / dgrad with streamed
+    // weights (128) or the column-sliced kernels (64), and wgrad
+    for (int bn : {64, 128}) {
+      for (auto c : fwd_combos) {
+        cs_conv_algo_desp f = conv_desp(0, rowgather_desp(c.in, c.w, c.out, cs::kMaskTile, bn, 0, 1, 0));
+        f.dynamic_mask = 1;
+        v.push_back(f);
+        cs_conv_algo_desp d = conv_desp(1, rowgather_desp(c.in, c.w, c.out, cs::kMaskTile, bn, 0, 0, 0));
+        d.dynamic_mask = 1;
+        v.push_back(d);
+      }
+    }
+    for (auto c : wg_combos) {
+      cs_conv_algo_desp w = conv_desp(2, wgrad_desp(c.in, c.w, c.out, cs::kMaskTile));
+      w.dynamic_mask = 1;
+      v.push_back(w);
+    }
     // int8 inference forward: int8 input / weight / output, int32 accumulation,
     // float epilogue act(scale * acc + bias + beta * output_add)
     for (int bn : {64, 32, 16, 128}) {
@@ -469,14 +486,14 @@ struct WgradLayout {
   int64_t partial = 0, rowmask = 0, bcount = 0, list_a = 0, list_b = 0, total = 0, size = 0;
 };
 
-WgradLayout wgrad_layout(int64_t kv, int64_t M, int64_t N, int64_t nrows, int splits, bool lists) {
+WgradLayout wgrad_layout(int64_t kv, int64_t M, int64_t N, int64_t nrows, int splits, bool lists, int mw = 1) {
   WgradLayout w;
   w.partial = 0;
   int64_t off = align256(kv * splits * M * N * 4);
   if (lists) {
     const int64_t nblk = std::max<int64_t>(1, (nrows + 4095) / 4096);
     w.rowmask = off;
-    off += align256(std::max<int64_t>(nrows, 1) * 4);
+    off += align256(std::max<int64_t>(nrows, 1) * 4 * mw);
     w.bcount = off;
     off += align256(kv * nblk * 4);
     w.list_a = off;
@@ -499,6 +516,7 @@ struct ConvPlan {
   int64_t N = 0, C = 0, K = 0, kv = 0;
   int64_t m = 0, n = 0, k = 0;
   int splits = 1;
+  int mw = 1;  // mask words per row (dynamic masks: div_up(kv, 32))
   int64_t workspace = 0;
 };
 
@@ -564,7 +582,15 @@ ConvPlan plan_conv(const cs_conv_params& p) {
   CS_ARG(plan.kernel != nullptr, "Can't Found Algorithm for params: ", conv_repr(desp));
 
   // Checks the generated kernels leave to the caller; here they guard memory safety.
-  CS_ARG(kv <= 32 && !desp.dynamic_mask, "kernel volume > 32 needs dynamic (multi-word) masks");
+  // Masks hold one 32-bit word per row, or div_up(kv, 32) words with dynamic
+  // masks (MaskIGemmIteratorMaskLoaderDynamic, cumm/gemm/blockmma/mma.py:78-269).
+  if (desp.dynamic_mask) {
+    CS_ARG(kv <= 32 * cs::kMaxMaskWords, "dynamic masks support kernel volumes up to ",
+           32 * cs::kMaxMaskWords, ", got ", kv);
+    plan.mw = (int)((kv + 31) / 32);
+  } else {
+    CS_ARG(kv <= 32, "kernel volume > 32 needs dynamic (multi-word) masks");
+  }
   const int dt_in = p.input.dtype, dt_w = p.weight.dtype, dt_out = p.output.dtype;
   const int abc[3] = {desp.gemm.dtype_a, desp.gemm.dtype_b, desp.gemm.dtype_c};
   // conv_iwo_012_to_abc (params.h:225-237): i/w/o -> a/b/c
@@ -585,14 +611,16 @@ ConvPlan plan_conv(const cs_conv_params& p) {
   check_index_dtype(p.mask_argsort, "mask_argsort");
   CS_RT(numel(p.mask_argsort) >= N, "mask_argsort must hold ", N, " entries");
   const int tile_m = desp.gemm.tile_shape[0];
+  const int64_t mw = plan.mw;
   if (desp.op_type == 2) {
-    CS_RT(numel(p.mask) >= (N + p.mask_width - 1) / p.mask_width, "mask must hold div_up(N, mask_width) entries");
+    CS_RT(numel(p.mask) >= (N + p.mask_width - 1) / p.mask_width * mw,
+          "mask must hold div_up(N, mask_width) * ", mw, " entries");
   } else {
-    CS_RT(numel(p.mask) >= N, "mask must hold ", N, " entries");
+    CS_RT(numel(p.mask) >= N * mw, "mask must hold ", N * mw, " entries");
   }
   if (desp.op_type == 0 && !empty(p.mask_output))
-    CS_RT(numel(p.mask_output) >= (N + tile_m - 1) / tile_m, "mask_output must hold div_up(N, ", tile_m,
-          ") entries");
+    CS_RT(numel(p.mask_output) >= (N + tile_m - 1) / tile_m * mw, "mask_output must hold div_up(N, ", tile_m,
+          ") * ", mw, " entries");
   const int dt_written = desp.op_type == 0 ? dt_out : desp.op_type == 1 ? dt_in : dt_w;
   if (!empty(p.bias)) {
     check_device(p.bias, "bias");
@@ -632,7 +660,7 @@ ConvPlan plan_conv(const cs_conv_params& p) {
   } else if (desp.op_type == 2) {
     CS_RT(K * C * 4 < kInt32Max, "your data exceed int32 range (wgrad output).");
     plan.splits = resolve_splits(p.split_k_slices, K, C);
-    plan.workspace = wgrad_layout(kv, K, C, N, plan.splits, true).size;
+    plan.workspace = wgrad_layout(kv, K, C, N, plan.splits, true, plan.mw).size;
   }
   return plan;
 }
@@ -680,6 +708,7 @@ void run_conv(const cs_conv_params& p) {
     a.out_row = a.pair_col;
     a.mask = reinterpret_cast<const uint32_t*>(p.mask.data);
     a.mask_out = plan.op == 0 && !empty(p.mask_output) ? reinterpret_cast<uint32_t*>(p.mask_output.data) : nullptr;
+    a.mw = plan.mw;
     a.scale = empty(p.scale) ? nullptr : reinterpret_cast<const float*>(p.scale.data);
     a.m = (int32_t)plan.N;
     a.kred = (int32_t)(plan.op == 0 ? plan.C : plan.K);
@@ -721,11 +750,13 @@ void run_conv(const cs_conv_params& p) {
     w.M = (int32_t)plan.K;
     w.N = (int32_t)plan.C;
     w.kv = (int32_t)plan.kv;
-    w.mask_filter = p.mask_filter & (plan.kv >= 32 ? 0xffffffffu : ((1u << plan.kv) - 1u));
+    // dynamic masks filter offset f by bit f % 32 (mma.py:611-614)
+    w.mask_filter = plan.mw > 1 ? p.mask_filter : p.mask_filter & (plan.kv >= 32 ? 0xffffffffu : ((1u << plan.kv) - 1u));
+    w.mw = plan.mw;
     w.max_blocks = plan.splits;
     w.min_rows = kWgradMinRows;
     w.fixed_total = -1;
-    const WgradLayout lay = wgrad_layout(plan.kv, plan.K, plan.C, plan.N, plan.splits, true);
+    const WgradLayout lay = wgrad_layout(plan.kv, plan.K, plan.C, plan.N, plan.splits, true, plan.mw);
     char* base = static_cast<char*>(ws.ptr);
     w.partial = reinterpret_cast<float*>(base + lay.partial);
     w.rowmask = reinterpret_cast<uint32_t*>(base + lay.rowmask);
@@ -899,6 +930,7 @@ void run_gemm(const cs_gemm_params& p) {
     w.N = (int32_t)g.n;
     w.kv = 1;
     w.mask_filter = 1u;
+    w.mw = 1;
     w.max_blocks = g.splits;
     w.min_rows = kWgradMinRows;
     const WgradLayout lay = wgrad_layout(1, g.m, g.n, g.k, g.splits, false);
@@ -970,6 +1002,7 @@ void run_gemm(const cs_gemm_params& p) {
   a.nout = (int32_t)g.n;
   a.kv = 1;
   a.mask_filter = 1u;
+  a.mw = 1;
   a.alpha = p.alpha;
   a.beta = p.beta;
   a.act_alpha = p.act_alpha;

@@ -134,6 +134,60 @@ template <> struct Mma<int8_t> {
 template <typename T> struct IsInt8 { static constexpr bool value = false; };
 template <> struct IsInt8<int8_t> { static constexpr bool value = true; };
 
+// ---------------------------------------------------------------- kernel-offset masks
+// Offset f of a kv-offset kernel is bit (f & 31) of word (f >> 5); kv <= 32
+// uses one word, larger kernels (dynamic masks) MW words per row.
+// Bits of word w that are offsets below kv.
+__device__ __forceinline__ uint32_t kv_word(int kv, int w) {
+  const int b = kv - 32 * w;
+  return b >= 32 ? 0xffffffffu : (b > 0 ? ((1u << b) - 1u) : 0u);
+}
+// Bit f of a multi-word mask (f uniform; no dynamic register indexing).
+template <int MW>
+__device__ __forceinline__ uint32_t mask_bit(const uint32_t (&m)[MW], int f) {
+  uint32_t v = m[0];
+#pragma unroll
+  for (int w = 1; w < MW; ++w) v = (f >> 5) == w ? m[w] : v;
+  return (v >> (f & 31)) & 1u;
+}
+// Lowest set offset above f (f = -1: the lowest), -1 if none.
+template <int MW>
+__device__ __forceinline__ int next_offset(const uint32_t (&m)[MW], int f) {
+  const int g = f + 1;
+#pragma unroll
+  for (int w = 0; w < MW; ++w) {
+    if (g < 32 * (w + 1)) {
+      const int b = g > 32 * w ? g - 32 * w : 0;
+      const uint32_t rest = m[w] & (0xffffffffu << b);
+      if (rest) return 32 * w + __builtin_ctz(rest);
+    }
+  }
+  return -1;
+}
+// Reversed kernel order (dgrad of a submanifold layer): bit f of the result is
+// bit kv - 1 - f of m.  Words past the mask are zero.
+template <int MW>
+__device__ __forceinline__ void reverse_kv(uint32_t (&m)[MW], int kv) {
+  if constexpr (MW == 1) {
+    m[0] = __builtin_bitreverse32(m[0]) >> (32 - kv);
+  } else {
+    uint32_t r[MW];
+#pragma unroll
+    for (int w = 0; w < MW; ++w) r[w] = __builtin_bitreverse32(m[MW - 1 - w]);
+    const int sh = 32 * MW - kv, q = sh >> 5, b = sh & 31;
+#pragma unroll
+    for (int w = 0; w < MW; ++w) {
+      uint32_t lo = 0u, hi = 0u;
+#pragma unroll
+      for (int j = 0; j < MW; ++j) {
+        lo = j == w + q ? r[j] : lo;
+        hi = j == w + q + 1 ? r[j] : hi;
+      }
+      m[w] = b ? ((lo >> b) | (hi << (32 - b))) : lo;
+    }
+  }
+}
+
 // ---------------------------------------------------------------- epilogue
 // D = act(alpha * acc + beta * source); act codes follow tv::gemm::Activation.
 enum : int { kActNone = 0, kActReLU = 1, kActSigmoid = 2, kActLeakyReLU = 3 };

@@ -31,8 +31,9 @@ struct RowGatherArgs {
   const int32_t* pairs; int64_t pair_ld;
   const int32_t* pair_col;
   const int32_t* out_row;
-  const uint32_t* mask;      // sorted per-row masks; null -> every offset of the filter
-  uint32_t* mask_out;        // per-16-row tile masks (forward), may be null
+  const uint32_t* mask;      // sorted per-row masks ([m][mw] words); null -> every offset of the filter
+  uint32_t* mask_out;        // per-16-row tile masks ([tiles][mw], forward), may be null
+  int32_t mw;                // mask words per row: 1 (kv <= 32) or div_up(kv, 32) <= 4 (dynamic masks)
   const float* scale;        // per-output-column multiplier, may be null
   // int8 inference (int8 a, b, c): c = act(scale[n] * acc + bias_f[n] + beta * d[o][n])
   const float* bias_f;
@@ -64,12 +65,13 @@ struct WgradArgs {
   const int32_t* a_rows;      // conv form: mask argsort
   const int32_t* pairs; int64_t pair_ld;
   const uint32_t* mask; int32_t mask_width; int32_t num_mask;
+  int32_t mw;                 // mask words per group / row (1, or div_up(kv, 32) for dynamic masks)
   int32_t nrows, M, N, kv;
   uint32_t mask_filter;
   int32_t max_blocks;        // pieces per offset (grid.z), <= kWgradMaxBlocks
   int32_t min_rows;          // pairs per piece, lower bound
   int32_t fixed_total;       // list form: entries of the given lists; else -1
-  uint32_t* rowmask;         // [nrows]
+  uint32_t* rowmask;         // [nrows][mw]
   int32_t* bcount;           // [kv][div_up(nrows, 4096)]
   int32_t* list_a;           // [kv][list_ld] (list form: a_inds, read only)
   int32_t* list_b;           // (list form: b_inds, read only)
@@ -117,6 +119,7 @@ hipError_t launch_scatter_add_f32(const float* in, float* out, const int32_t* id
 constexpr int kRowsPerWave = 64;     // four 16-row mask tiles per wave
 constexpr int kWavesPerBlock = 4;
 constexpr int kMaskTile = 16;        // rows per kernel-offset mask tile
+constexpr int kMaxMaskWords = 4;     // dynamic masks: kv <= 128
 constexpr int kWgradRows = 64;       // rows staged per wgrad step
 constexpr int kWgradMaxBlocks = 128; // pieces per offset, upper bound
 constexpr int kWgradTile = 64;       // wgrad output tile (M and N)

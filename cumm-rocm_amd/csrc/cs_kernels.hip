@@ -63,12 +63,13 @@ __global__ __launch_bounds__(256) void wgrad_rowmask_kernel(WgradArgs p) {
   const int r = blockIdx.x * 256 + threadIdx.x;
   if (r >= p.nrows) return;
   const int i = p.a_rows[r];
-  uint32_t m = p.mask_filter;
-  if (p.mask != nullptr) {
-    const int mi = (int)((int64_t)r / p.mask_width);
-    m &= mi < p.num_mask ? p.mask[mi] : 0u;
+  if ((unsigned)i >= (unsigned)p.nrows) return;
+  const int mi = (int)((int64_t)r / p.mask_width);
+  for (int w = 0; w < p.mw; ++w) {
+    uint32_t m = p.mask_filter;
+    if (p.mask != nullptr) m &= mi < p.num_mask ? p.mask[(int64_t)mi * p.mw + w] : 0u;
+    p.rowmask[(int64_t)i * p.mw + w] = m;
   }
-  if ((unsigned)i < (unsigned)p.nrows) p.rowmask[i] = m;
 }
 
 // Pair-list construction, two passes over [kv] x [blocks of kWgradListRows
@@ -82,20 +83,22 @@ constexpr int kWgradListRows = 4096;
 // returns the bits of the rows that hold a pair for offset f.
 __device__ __forceinline__ uint32_t wgrad_rows(const WgradArgs& p, int f, int i0, int tid, int (&jv)[16]) {
   const __amdgpu_buffer_rsrc_t rp = make_rsrc(p.pairs + (int64_t)f * p.pair_ld, (uint32_t)p.nrows * 4u);
-  const __amdgpu_buffer_rsrc_t rm = make_rsrc(p.rowmask, (uint32_t)p.nrows * 4u);
+  // word f / 32 of each row's mask: rowmask is [nrows][mw]
+  const __amdgpu_buffer_rsrc_t rm = make_rsrc(p.rowmask + (f >> 5), (uint32_t)p.nrows * (uint32_t)p.mw * 4u);
   uint32_t mv[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     const int i = i0 + tid + 256 * k;
     const uint32_t off = i < p.nrows ? (uint32_t)i * 4u : kBufOOB;
     jv[k] = (int)__builtin_amdgcn_raw_buffer_load_b32(rp, off, 0, 0);
-    mv[k] = __builtin_amdgcn_raw_buffer_load_b32(rm, off, 0, 0);
+    const uint32_t moff = i < p.nrows ? (uint32_t)i * (uint32_t)p.mw * 4u : kBufOOB;
+    mv[k] = __builtin_amdgcn_raw_buffer_load_b32(rm, moff, 0, 0);
   }
   uint32_t bits = 0;
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     const int i = i0 + tid + 256 * k;
-    bits |= (uint32_t)(i < p.nrows && jv[k] >= 0 && ((mv[k] >> f) & 1u)) << k;
+    bits |= (uint32_t)(i < p.nrows && jv[k] >= 0 && ((mv[k] >> (f & 31)) & 1u)) << k;
   }
   return bits;
 }
@@ -106,7 +109,7 @@ __global__ __launch_bounds__(256) void wgrad_count_kernel(WgradArgs p) {
   const int tid = threadIdx.x;
   __shared__ int wsum[4];
   int cnt = 0;
-  if ((p.mask_filter >> f) & 1u) {
+  if ((p.mask_filter >> (f & 31)) & 1u) {
     int jv[16];
     cnt = __popc(wgrad_rows(p, f, i0, tid, jv));
   }
@@ -125,7 +128,7 @@ __global__ __launch_bounds__(256) void wgrad_list_kernel(WgradArgs p) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   __shared__ int wsum[16][4];
   __shared__ int sbase;
-  if (!((p.mask_filter >> f) & 1u)) {
+  if (!((p.mask_filter >> (f & 31)) & 1u)) {
     if (blk == 0 && tid == 0) p.total[f] = 0;
     return;
   }
@@ -191,7 +194,7 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs p) {
   __shared__ __attribute__((aligned(16))) char lds_b[TN * PITCH];
 
   const int f = blockIdx.y;
-  if (!((p.mask_filter >> f) & 1u)) return;
+  if (!((p.mask_filter >> (f & 31)) & 1u)) return;
   const int count = wgrad_total(p, f);
   const int nb = wgrad_pieces(count, p.max_blocks, p.min_rows);
   const int piece = blockIdx.z;
@@ -322,7 +325,7 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(ReduceArgs p) {
   const bool live = e < (int64_t)p.kv * mn;
   const int fe = live ? (int)(e / mn) : 0;
   const int64_t rem = e - (int64_t)fe * mn;
-  const bool on = live && ((p.mask_filter >> fe) & 1u);
+  const bool on = live && ((p.mask_filter >> (fe & 31)) & 1u);
   const int nb = on ? wgrad_pieces(p.fixed_total >= 0 ? p.fixed_total : p.total[fe], p.max_blocks, p.min_rows) : 0;
   const __amdgpu_buffer_rsrc_t rp = make_rsrc(p.partial + (int64_t)fe * p.max_blocks * mn,
                                               (uint32_t)((int64_t)nb * mn * 4));
@@ -363,7 +366,7 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(ReduceArgs p) {
   const int64_t eo = (int64_t)blockIdx.x * 16 * V + tid;
   if (eo >= (int64_t)p.kv * mn) return;
   const int fo = (int)(eo / mn);
-  if (!((p.mask_filter >> fo) & 1u)) return;
+  if (!((p.mask_filter >> (fo & 31)) & 1u)) return;
   float t[PQ];
 #pragma unroll
   for (int i = 0; i < PQ; ++i) t[i] = red[i][tid];

@@ -35,7 +35,8 @@ constexpr int kRgRedPitch = 68;   // LDS floats per row of a partial tile
 // HAS_PAIRS: gather through a pair table (else the identity / a_inds-free GEMM).
 // FULL: kred is a multiple of the per-pass reduction width and nout a multiple
 // of 64, so no lane needs a reduction- or column-range predicate.
-template <typename T, typename OutT, bool HAS_PAIRS, bool FULL>
+// MW: mask words handled (1, or kMaxMaskWords for dynamic masks, kv > 32).
+template <typename T, typename OutT, bool HAS_PAIRS, bool FULL, int MW>
 __global__ __launch_bounds__(256) void rowgather_kernel(RowGatherArgs p) {
   using MM = Mma<T>;
   constexpr int E = MM::kElems;
@@ -62,43 +63,52 @@ __global__ __launch_bounds__(256) void rowgather_kernel(RowGatherArgs p) {
 
   const int myrow = row0 + lane;
   const bool rvalid = myrow < p.m;
-  const uint32_t kvmask = p.kv >= 32 ? 0xffffffffu : ((1u << p.kv) - 1u);
-  uint32_t mymask = 0;
   int mycol = 0, myout = 0;
   if (rvalid) {
-    mymask = p.mask ? p.mask[myrow] : kvmask;
     mycol = p.pair_col ? p.pair_col[myrow] : myrow;
     myout = p.out_row ? p.out_row[myrow] : myrow;
   }
-  // OR of the 16 row masks of each tile (lanes 16s..16s+15 hold tile s).
-  uint32_t g = mymask;
-  g |= __shfl_xor(g, 1);
-  g |= __shfl_xor(g, 2);
-  g |= __shfl_xor(g, 4);
-  g |= __shfl_xor(g, 8);
-  uint32_t sm[4];
+  // OR of the 16 row masks of each tile (lanes 16s..16s+15 hold tile s), word by word.
+  uint32_t sm[4][MW];
 #pragma unroll
-  for (int s = 0; s < 4; ++s) sm[s] = __builtin_amdgcn_readlane(g, 16 * s) & p.mask_filter & kvmask;
-
-  if (wave == 0 && p.mask_out != nullptr && lane < 4 && row0 + 16 * lane < p.m) {
-    const uint32_t v = lane == 0 ? sm[0] : lane == 1 ? sm[1] : lane == 2 ? sm[2] : sm[3];
-    p.mask_out[row0 / kMaskTile + lane] = v;
+  for (int w = 0; w < MW; ++w) {
+    uint32_t g = 0u;
+    if (rvalid && w < p.mw) g = p.mask ? p.mask[(int64_t)myrow * p.mw + w] : 0xffffffffu;
+    g |= __shfl_xor(g, 1);
+    g |= __shfl_xor(g, 2);
+    g |= __shfl_xor(g, 4);
+    g |= __shfl_xor(g, 8);
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+      sm[s][w] = __builtin_amdgcn_readlane(g, 16 * s) & p.mask_filter & kv_word(p.kv, w);
+    if (wave == 0 && p.mask_out != nullptr && lane < 4 && row0 + 16 * lane < p.m && w < p.mw) {
+      const uint32_t v = lane == 0 ? sm[0][w] : lane == 1 ? sm[1][w] : lane == 2 ? sm[2][w] : sm[3][w];
+      p.mask_out[(int64_t)(row0 / kMaskTile + lane) * p.mw + w] = v;
+    }
   }
   uint32_t live = 0;
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
-    if (row0 + 16 * s < p.m && (!p.skip_empty || sm[s] != 0u)) live |= 1u << s;
+    uint32_t any = 0u;
+#pragma unroll
+    for (int w = 0; w < MW; ++w) any |= sm[s][w];
+    if (row0 + 16 * s < p.m && (!p.skip_empty || any != 0u)) live |= 1u << s;
   }
   if (p.reverse_mask) {
 #pragma unroll
-    for (int s = 0; s < 4; ++s) sm[s] = __builtin_bitreverse32(sm[s]) >> (32 - p.kv);
+    for (int s = 0; s < 4; ++s) reverse_kv(sm[s], p.kv);
   }
-  const uint32_t uni = (sm[0] | sm[1] | sm[2] | sm[3]);
   // This wave's share: offsets f with f % 4 == wave.  The split depends on the
   // offset index only, never on which offsets the tile's union holds, so every
   // row's partial sums (and hence its bits) are the same whatever rows share
   // its tile: a cloud gives identical rows alone and inside a batch.
-  const uint32_t mine = uni & (0x11111111u << wave);
+  uint32_t mine[MW];
+  uint32_t mine_any = 0u;
+#pragma unroll
+  for (int w = 0; w < MW; ++w) {
+    mine[w] = (sm[0][w] | sm[1][w] | sm[2][w] | sm[3][w]) & (0x11111111u << wave);
+    mine_any |= mine[w];
+  }
 
   int cols[4];
 #pragma unroll
@@ -123,7 +133,6 @@ __global__ __launch_bounds__(256) void rowgather_kernel(RowGatherArgs p) {
   const __amdgpu_buffer_rsrc_t rb = make_rsrc(p.b, p.b_bytes);
   auto load_frags = [&](int f, const int (&idx)[4], int kb, u32x4 (&af)[4][KCH], u32x4 (&bfr)[KCH][NB]) {
     const bool fvalid = f >= 0;
-    const uint32_t fbit = fvalid ? (1u << f) : 0u;
     const int64_t fb = (int64_t)(fvalid ? f : 0) * p.b_fs;
 #pragma unroll
     for (int kc = 0; kc < KCH; ++kc) {
@@ -138,7 +147,7 @@ __global__ __launch_bounds__(256) void rowgather_kernel(RowGatherArgs p) {
       }
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
-        const bool on = kin && (sm[s] & fbit) != 0u && idx[s] >= 0;
+        const bool on = kin && mask_bit(sm[s], f) != 0u && idx[s] >= 0;
         const uint32_t off = on ? (uint32_t)(((int64_t)idx[s] * p.lda + k) * (int64_t)sizeof(T)) : kBufOOB;
         af[s][kc] = bld16(ra, off);
       }
@@ -147,7 +156,7 @@ __global__ __launch_bounds__(256) void rowgather_kernel(RowGatherArgs p) {
   auto compute = [&](int f, const u32x4 (&af)[4][KCH], const u32x4 (&bfr)[KCH][NB]) {
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-      if ((sm[s] >> f) & 1u) {
+      if (mask_bit(sm[s], f)) {
 #pragma unroll
         for (int kc = 0; kc < KCH; ++kc)
 #pragma unroll
@@ -159,11 +168,21 @@ __global__ __launch_bounds__(256) void rowgather_kernel(RowGatherArgs p) {
   // This wave's offsets, two at a time: a pair's fragment loads form one
   // memory round trip, and the next pair's indices load while the MFMAs run.
   for (int kb = 0; kb < p.kred; kb += KCH * CH) {
-    if (!mine) break;
-    uint32_t rem = mine;
-    auto pop = [&]() -> int {
-      const int f = rem ? __builtin_ctz(rem) : -1;
-      rem &= rem - 1u;
+    if (!mine_any) break;
+    uint32_t rem[MW];
+#pragma unroll
+    for (int w = 0; w < MW; ++w) rem[w] = mine[w];
+    auto pop = [&]() -> int {  // lowest remaining offset, in ascending order
+      int f = -1;
+#pragma unroll
+      for (int w = MW - 1; w >= 0; --w) {
+        if (rem[w]) f = 32 * w + __builtin_ctz(rem[w]);
+      }
+      if (f >= 0) {
+#pragma unroll
+        for (int w = 0; w < MW; ++w)
+          if ((f >> 5) == w) rem[w] &= rem[w] - 1u;
+      }
       return f;
     };
     int fa = pop(), fb = pop();
@@ -242,49 +261,71 @@ static int ws_slices(const RowGatherArgs& a) {
   return ncol > 0 ? (a.nout + ncol - 1) / ncol : INT_MAX;
 }
 
-template <typename T, typename OutT>
-static hipError_t launch_rg(const RowGatherArgs& a, int block_n, hipStream_t s) {
-  if (block_n != 16 && block_n != 32 && block_n != 64 && block_n != 128) return hipErrorInvalidValue;
-  bool used = false;
-  hipError_t e;
-  // measured on MI355X: two slices beat streaming for 1- and 2-byte types, four for f32
-  const bool prefer_stream = block_n == 128 || (block_n == 64 && ws_slices<T>(a) > (sizeof(T) == 4 ? 4 : 2));
-  if (!prefer_stream) {
-    e = launch_ws<T, OutT>(a, block_n, s, &used);
-    if (used || e != hipSuccess) return e;
-  }
-  if (a.b_kmajor) {
-    // the other kernels read B n-major: transpose [kred][kv][nout] -> [kv][nout][kred]
-    if (a.b_scratch == nullptr) return hipErrorInvalidValue;
-    const hipError_t et = launch_transpose(a.b, a.b_scratch, a.kred, a.kv, a.nout, a.b_ks, (int)sizeof(T), s);
-    if (et != hipSuccess) return et;
-    RowGatherArgs a2 = a;
-    a2.b = a.b_scratch;
-    a2.b_kmajor = 0;
-    a2.b_fs = (int64_t)a.nout * a.kred;
-    a2.b_ns = a.kred;
-    a2.b_bytes = (uint32_t)((int64_t)a.kv * a.nout * a.kred * (int64_t)sizeof(T));
-    return launch_rg<T, OutT>(a2, block_n, s);
-  }
-  e = launch_stream<T, OutT>(a, s, &used);
-  if (used || e != hipSuccess) return e;
-  if (prefer_stream) {
-    e = launch_ws<T, OutT>(a, 64, s, &used);
-    if (used || e != hipSuccess) return e;
-  }
+template <typename T, typename OutT, int MW>
+static hipError_t launch_block(const RowGatherArgs& a, hipStream_t s) {
   dim3 grid((a.m + kRgRows - 1) / kRgRows, (a.nout + 63) / 64, 1);
   dim3 block(256, 1, 1);
   if (grid.x == 0 || grid.y == 0) return hipSuccess;
   constexpr int pass = RgChunks<T>::value * Mma<T>::kChunk;
   const bool full = a.kred % pass == 0 && a.nout % 64 == 0;
   if (a.pairs) {
-    if (full) hipLaunchKernelGGL((rowgather_kernel<T, OutT, true, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((rowgather_kernel<T, OutT, true, false>), grid, block, 0, s, a);
+    if (full) hipLaunchKernelGGL((rowgather_kernel<T, OutT, true, true, MW>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((rowgather_kernel<T, OutT, true, false, MW>), grid, block, 0, s, a);
+  } else if constexpr (MW == 1) {
+    if (full) hipLaunchKernelGGL((rowgather_kernel<T, OutT, false, true, 1>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((rowgather_kernel<T, OutT, false, false, 1>), grid, block, 0, s, a);
   } else {
-    if (full) hipLaunchKernelGGL((rowgather_kernel<T, OutT, false, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((rowgather_kernel<T, OutT, false, false>), grid, block, 0, s, a);
+    return hipErrorInvalidValue;  // multi-word masks come with a pair table
   }
   return hipGetLastError();
+}
+
+// B_f n-major copy of k-major weights into b_scratch, then the launch on it.
+template <typename T, typename OutT>
+static hipError_t launch_rg(const RowGatherArgs& a, int block_n, hipStream_t s);
+
+template <typename T, typename OutT>
+static hipError_t launch_transposed(const RowGatherArgs& a, int block_n, hipStream_t s) {
+  // the other kernels read B n-major: transpose [kred][kv][nout] -> [kv][nout][kred]
+  if (a.b_scratch == nullptr) return hipErrorInvalidValue;
+  const hipError_t et = launch_transpose(a.b, a.b_scratch, a.kred, a.kv, a.nout, a.b_ks, (int)sizeof(T), s);
+  if (et != hipSuccess) return et;
+  RowGatherArgs a2 = a;
+  a2.b = a.b_scratch;
+  a2.b_kmajor = 0;
+  a2.b_fs = (int64_t)a.nout * a.kred;
+  a2.b_ns = a.kred;
+  a2.b_bytes = (uint32_t)((int64_t)a.kv * a.nout * a.kred * (int64_t)sizeof(T));
+  return launch_rg<T, OutT>(a2, block_n, s);
+}
+
+template <typename T, typename OutT>
+static hipError_t launch_rg(const RowGatherArgs& a, int block_n, hipStream_t s) {
+  if (block_n != 16 && block_n != 32 && block_n != 64 && block_n != 128) return hipErrorInvalidValue;
+  if (a.mw < 1 || a.mw > kMaxMaskWords) return hipErrorInvalidValue;
+  bool used = false;
+  hipError_t e;
+  if (a.mw > 1) {
+    // dynamic masks (kv > 32): streamed weights where they apply, else the block kernel
+    if (a.b_kmajor) return launch_transposed<T, OutT>(a, block_n, s);
+    e = launch_stream<T, OutT, kMaxMaskWords>(a, s, &used);
+    if (used || e != hipSuccess) return e;
+    return launch_block<T, OutT, kMaxMaskWords>(a, s);
+  }
+  // measured on MI355X: two slices beat streaming for 1- and 2-byte types, four for f32
+  const bool prefer_stream = block_n == 128 || (block_n == 64 && ws_slices<T>(a) > (sizeof(T) == 4 ? 4 : 2));
+  if (!prefer_stream) {
+    e = launch_ws<T, OutT>(a, block_n, s, &used);
+    if (used || e != hipSuccess) return e;
+  }
+  if (a.b_kmajor) return launch_transposed<T, OutT>(a, block_n, s);
+  e = launch_stream<T, OutT, 1>(a, s, &used);
+  if (used || e != hipSuccess) return e;
+  if (prefer_stream) {
+    e = launch_ws<T, OutT>(a, 64, s, &used);
+    if (used || e != hipSuccess) return e;
+  }
+  return launch_block<T, OutT, 1>(a, s);
 }
 
 }  // namespace cs

@@ -23,8 +23,9 @@ namespace cs {
 constexpr int kStWaves = 8;                 // waves per block = mask tiles per block
 constexpr int kStLdsBytes = 2 * 32768;      // two weight stages of up to 32 KiB
 
-// KC: reduction chunks (kred / chunk); NB: 16-column blocks (nout <= 16 NB)
-template <typename T, typename OutT, int KC, int NB>
+// KC: reduction chunks (kred / chunk); NB: 16-column blocks (nout <= 16 NB);
+// MW: mask words handled (1, or kMaxMaskWords for dynamic masks, kv > 32)
+template <typename T, typename OutT, int KC, int NB, int MW>
 __global__ __launch_bounds__(64 * kStWaves) void rowgather_stream_kernel(RowGatherArgs p, int ntiles) {
   using MM = Mma<T>;
   constexpr int E = MM::kElems;
@@ -38,41 +39,56 @@ __global__ __launch_bounds__(64 * kStWaves) void rowgather_stream_kernel(RowGath
   constexpr int NT = 64 * kStWaves;
   constexpr int WCH = (NCOL * CHUNKS + NT - 1) / NT;  // 16-byte weight chunks per thread
   __shared__ __attribute__((aligned(16))) char wlds[kStLdsBytes];
-  __shared__ uint32_t wmask[kStWaves];
+  __shared__ uint32_t wmask[kStWaves][MW];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int lr = lane & 15, lg = lane >> 4;
-  const uint32_t kvmask = p.kv >= 32 ? 0xffffffffu : ((1u << p.kv) - 1u);
   const int t = blockIdx.x * kStWaves + wave;
   const int row0 = t * kMaskTile;
   const int r = row0 + lr;
   const bool rvalid = t < ntiles && r < p.m;
 
-  // ---- this wave's tile: masks, pair column, output row of row lr
-  uint32_t m = 0u;
+  // ---- this wave's tile: masks (word by word), pair column, output row of row lr
   int col = 0, orow = 0;
   if (rvalid) {
-    m = p.mask ? p.mask[r] : kvmask;
     col = p.pair_col ? p.pair_col[r] : r;
     orow = p.out_row ? p.out_row[r] : r;
   }
-  m |= __shfl_xor(m, 1);
-  m |= __shfl_xor(m, 2);
-  m |= __shfl_xor(m, 4);
-  m |= __shfl_xor(m, 8);
-  uint32_t km = __builtin_amdgcn_readfirstlane(m) & p.mask_filter & kvmask;
-  if (t < ntiles && p.mask_out != nullptr && lane == 0) p.mask_out[t] = km;
-  const bool live = t < ntiles && !(p.skip_empty && km == 0u);
-  if (!live) km = 0u;
-  if (p.reverse_mask && km) km = __builtin_bitreverse32(km) >> (32 - p.kv);
-  if (lane == 0) wmask[wave] = km;
-  __syncthreads();
-  uint32_t U = 0u;
+  uint32_t km[MW];
+  uint32_t any = 0u;
 #pragma unroll
-  for (int w = 0; w < kStWaves; ++w) U |= wmask[w];
-  U = __builtin_amdgcn_readfirstlane(U);
+  for (int w = 0; w < MW; ++w) {
+    uint32_t m = 0u;
+    if (rvalid && w < p.mw) m = p.mask ? p.mask[(int64_t)r * p.mw + w] : 0xffffffffu;
+    m |= __shfl_xor(m, 1);
+    m |= __shfl_xor(m, 2);
+    m |= __shfl_xor(m, 4);
+    m |= __shfl_xor(m, 8);
+    km[w] = __builtin_amdgcn_readfirstlane(m) & p.mask_filter & kv_word(p.kv, w);
+    if (t < ntiles && w < p.mw && p.mask_out != nullptr && lane == 0) p.mask_out[(int64_t)t * p.mw + w] = km[w];
+    any |= km[w];
+  }
+  const bool live = t < ntiles && !(p.skip_empty && any == 0u);
+  if (!live) {
+#pragma unroll
+    for (int w = 0; w < MW; ++w) km[w] = 0u;
+  }
+  if (p.reverse_mask && any) reverse_kv(km, p.kv);
+  if (lane == 0) {
+#pragma unroll
+    for (int w = 0; w < MW; ++w) wmask[wave][w] = km[w];
+  }
+  __syncthreads();
+  uint32_t U[MW];
+#pragma unroll
+  for (int w = 0; w < MW; ++w) {
+    uint32_t u = 0u;
+#pragma unroll
+    for (int v = 0; v < kStWaves; ++v) u |= wmask[v][w];
+    U[w] = __builtin_amdgcn_readfirstlane(u);
+  }
 
   const __amdgpu_buffer_rsrc_t ra = make_rsrc(p.a, p.a_bytes);
   const __amdgpu_buffer_rsrc_t rb = make_rsrc(p.b, p.b_bytes);
@@ -116,19 +132,15 @@ __global__ __launch_bounds__(64 * kStWaves) void rowgather_stream_kernel(RowGath
       af[kc] = bld16(ra, off);
     }
   };
-  auto next_bit = [](uint32_t u, int f) -> int {  // lowest set bit above f (f = -1: lowest)
-    const uint32_t rest = f >= 31 ? 0u : (u & (0xffffffffu << (f + 1)));
-    return rest ? __builtin_ctz(rest) : -1;
-  };
 
   f32x4 acc[NB];
 #pragma unroll
   for (int nb = 0; nb < NB; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
 
   // ---- pipeline over the block's offsets f0 < f1 < f2 < ...
-  const int f0 = next_bit(U, -1);
+  const int f0 = next_offset(U, -1);
   if (f0 >= 0) {
-    const int f1 = next_bit(U, f0);
+    const int f1 = next_offset(U, f0);
     u32x4 wv[WCH];
     load_w(f0, wv);
     int i0 = load_idx(f0);
@@ -140,12 +152,12 @@ __global__ __launch_bounds__(64 * kStWaves) void rowgather_stream_kernel(RowGath
     __syncthreads();
     int fc = f0, fn = f1, buf = 0;
     while (fc >= 0) {
-      const int fnn = fn >= 0 ? next_bit(U, fn) : -1;
+      const int fnn = fn >= 0 ? next_offset(U, fn) : -1;
       // rows of the next offset and pair indices of the one after
       u32x4 a_nxt[KC];
       gather(i1, a_nxt);
       const int i2 = load_idx(fnn);
-      if ((km >> fc) & 1u) {
+      if (mask_bit(km, fc)) {
         const char* wb = wlds + buf * STAGE;
 #pragma unroll
         for (int kc = 0; kc < KC; ++kc) {
@@ -217,7 +229,7 @@ __global__ __launch_bounds__(64 * kStWaves) void rowgather_stream_kernel(RowGath
 // Launches the streamed-weights kernel when it applies (n-major weights, the
 // reduction a multiple of the MFMA chunk with at most 4 chunks, at most 128
 // output columns); *used = false otherwise.
-template <typename T, typename OutT>
+template <typename T, typename OutT, int MW>
 static hipError_t launch_stream(const RowGatherArgs& a, hipStream_t s, bool* used) {
   *used = false;
   constexpr int CH = Mma<T>::kChunk;
@@ -233,7 +245,7 @@ static hipError_t launch_stream(const RowGatherArgs& a, hipStream_t s, bool* use
   }
   dim3 grid((unsigned)((ntiles + kStWaves - 1) / kStWaves)), block(64 * kStWaves);
   *used = true;
-#define CS_ST(KC_, NB_) hipLaunchKernelGGL((rowgather_stream_kernel<T, OutT, KC_, NB_>), grid, block, 0, s, a, ntiles)
+#define CS_ST(KC_, NB_) hipLaunchKernelGGL((rowgather_stream_kernel<T, OutT, KC_, NB_, MW>), grid, block, 0, s, a, ntiles)
   switch (kc * 16 + nb) {
     case 1 * 16 + 2: CS_ST(1, 2); break;
     case 1 * 16 + 4: CS_ST(1, 4); break;

@@ -73,17 +73,19 @@ _CATALOGUE: List[ConvAlgoDesp] = []
 
 
 def select_conv_algo(op_type: int, dtype_input: int, dtype_weight: int, dtype_output: int,
-                     C: int, K: int) -> ConvAlgoDesp:
+                     C: int, K: int, dynamic_mask: bool = False) -> ConvAlgoDesp:
     """Pick this build's kernel for a conv layer, the way a caller filters
-    ``get_all_conv_algo_desp()`` by op and dtypes (64 output columns per wave).
-    Returns a fresh copy; the catalogue is read from the library once."""
+    ``get_all_conv_algo_desp()`` by op, dtypes and mask kind (64 output
+    columns per wave); ``dynamic_mask`` selects the multi-word-mask kernels
+    that kernel volumes above 32 need.  Returns a fresh copy; the catalogue is
+    read from the library once."""
     if not _CATALOGUE:
         _CATALOGUE.extend(ConvMainUnitTest.get_all_conv_algo_desp())
     op = int(op_type)
     bn = 64
     best = None
     for d in _CATALOGUE:
-        if int(d.op_type) != op:
+        if int(d.op_type) != op or bool(d.dynamic_mask) != bool(dynamic_mask):
             continue
         if (d.dtype_input, d.dtype_weight, d.dtype_output) != (dtype_input, dtype_weight, dtype_output):
             continue

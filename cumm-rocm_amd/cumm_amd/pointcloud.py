@@ -212,13 +212,16 @@ def strided_pairs(in_coords: np.ndarray, in_shape: Sequence[int], out_coords: np
 # masks and argsort
 # --------------------------------------------------------------------------
 def row_masks(pair: np.ndarray) -> np.ndarray:
-    """``mask[i]`` with bit ``f`` set iff ``pair[f][i] != -1`` (uint32, kv <= 32)."""
+    """``mask[i]`` with bit ``f`` set iff ``pair[f][i] != -1``: uint32 [N] for
+    kv <= 32, else the dynamic-mask layout [N, div_up(kv, 32)] with offset f in
+    bit f % 32 of word f // 32 (cumm/gemm/blockmma/mma.py:168-199)."""
     kv = pair.shape[0]
-    assert kv <= 32, "static masks hold at most 32 kernel offsets"
-    m = np.zeros(pair.shape[1], dtype=np.uint64)
+    mw = (kv + 31) // 32
+    m = np.zeros((pair.shape[1], mw), dtype=np.uint64)
     for f in range(kv):
-        m |= (pair[f] >= 0).astype(np.uint64) << np.uint64(f)
-    return m.astype(np.uint32)
+        m[:, f // 32] |= (pair[f] >= 0).astype(np.uint64) << np.uint64(f % 32)
+    m = m.astype(np.uint32)
+    return m[:, 0].copy() if kv <= 32 else m
 
 
 def sort_masks(mask: np.ndarray, split_mask: int = 0xFFFFFFFF) -> Tuple[np.ndarray, np.ndarray]:
@@ -230,7 +233,11 @@ def sort_masks(mask: np.ndarray, split_mask: int = 0xFFFFFFFF) -> Tuple[np.ndarr
     each launch only sees its own offsets, main_real_spconv.py:127-143).
     """
     key = (mask.astype(np.uint64) & np.uint64(split_mask)).astype(np.uint32)
-    order = np.argsort(key, kind="stable").astype(np.int32)
+    if key.ndim == 1:
+        order = np.argsort(key, kind="stable").astype(np.int32)
+    else:
+        # multi-word masks: stable sort by the words as one number, last word first
+        order = np.lexsort([key[:, w] for w in range(key.shape[1])]).astype(np.int32)
     return key[order], order
 
 
@@ -241,8 +248,8 @@ class SubmProblem:
     spatial_shape: Tuple[int, int, int]
     pair_fwd: np.ndarray        # [kv, N] int32
     pair_bwd: np.ndarray        # [kv, N] int32
-    mask: np.ndarray            # [N] uint32 (original order)
-    mask_sorted: np.ndarray     # [N] uint32 (sorted order)
+    mask: np.ndarray            # [N] uint32 (original order); [N, W] for kv > 32
+    mask_sorted: np.ndarray     # [N] uint32 (sorted order); [N, W] for kv > 32
     argsort: np.ndarray         # [N] int32
 
     @property
@@ -303,14 +310,16 @@ def tile_masks(mask_sorted: np.ndarray, tile_m: int, mask_filter: int = 0xFFFFFF
     forward kernel stores in ``mask_output`` (cumm/conv/kernel.py:843-874)."""
     n = mask_sorted.shape[0]
     t = (n + tile_m - 1) // tile_m
-    pad = np.zeros(t * tile_m, dtype=np.uint32)
-    pad[:n] = mask_sorted
-    return (np.bitwise_or.reduce(pad.reshape(t, tile_m), axis=1) & np.uint32(mask_filter)).astype(np.uint32)
+    words = mask_sorted.reshape(n, -1)
+    pad = np.zeros((t * tile_m, words.shape[1]), dtype=np.uint32)
+    pad[:n] = words
+    red = (np.bitwise_or.reduce(pad.reshape(t, tile_m, -1), axis=1) & np.uint32(mask_filter)).astype(np.uint32)
+    return red[:, 0].copy() if mask_sorted.ndim == 1 else red
 
 
 def mac_efficiency(mask_sorted: np.ndarray, tile_m: int, num_pairs: int) -> float:
     """``P / sum_t tile_m * popcount(tile_mask_t)``: fraction of issued MACs that are real."""
     tm = tile_masks(mask_sorted, tile_m)
-    pc = np.array([bin(int(v)).count("1") for v in tm], dtype=np.int64)
+    pc = np.array([bin(int(v)).count("1") for v in tm.reshape(-1)], dtype=np.int64)
     issued = int(pc.sum()) * tile_m
     return num_pairs / issued if issued else 1.0

@@ -86,19 +86,48 @@ def int8_inference_epilogue(acc, scale, bias, beta: float = 0.0, source=None, ac
 
 def tile_masks(mask_sorted: np.ndarray, n: int, tile_m: int, mask_filter: int = ALL) -> np.ndarray:
     """Per-tile OR of the sorted row masks AND mask_filter
-    (cumm/conv/kernel.py:843-866; rows past N contribute 0)."""
+    (cumm/conv/kernel.py:843-866; rows past N contribute 0).  Masks are [N]
+    (kv <= 32) or [N, W] words for dynamic masks, where the same 32-bit filter
+    applies to every word (MaskIGemmIteratorMaskLoaderDynamic::load_mask,
+    cumm/gemm/blockmma/mma.py:168-199); the result has the input's rank."""
     t = (n + tile_m - 1) // tile_m
-    pad = np.zeros(t * tile_m, dtype=np.uint64)
-    pad[:n] = mask_sorted[:n].astype(np.uint64)
-    return (np.bitwise_or.reduce(pad.reshape(t, tile_m), axis=1) & np.uint64(mask_filter)).astype(np.uint64)
+    m = np.asarray(mask_sorted)
+    words = m.reshape(m.shape[0], -1)[:n].astype(np.uint64)
+    pad = np.zeros((t * tile_m, words.shape[1]), dtype=np.uint64)
+    pad[:n] = words
+    red = np.bitwise_or.reduce(pad.reshape(t, tile_m, -1), axis=1) & np.uint64(mask_filter)
+    return red[:, 0] if m.ndim == 1 else red
+
+
+def mask_bits(masks: np.ndarray, kv: int) -> np.ndarray:
+    """[R] or [R, W] mask words -> bool [R, kv]: offset f is bit f % 32 of word f // 32."""
+    m = np.asarray(masks).astype(np.uint64)
+    m = m.reshape(m.shape[0], -1)
+    f = np.arange(kv)
+    return ((m[:, f // 32] >> (f % 32).astype(np.uint64)) & np.uint64(1)).astype(bool)
+
+
+def _pack(bits: np.ndarray, like: np.ndarray) -> np.ndarray:
+    """bool [R, kv] -> words shaped like ``like`` ([R] or [R, W])."""
+    w = 1 if like.ndim == 1 else like.shape[1]
+    out = np.zeros((bits.shape[0], w), dtype=np.uint64)
+    for f in range(bits.shape[1]):
+        out[:, f // 32] |= bits[:, f].astype(np.uint64) << np.uint64(f % 32)
+    return out[:, 0] if like.ndim == 1 else out
 
 
 def _brev_kv(m: np.ndarray, kv: int) -> np.ndarray:
-    """``__brev(kmask) >> (32 - kv)`` (cumm/conv/kernel.py:883-889)."""
-    out = np.zeros_like(m)
-    for f in range(kv):
-        out |= ((m >> np.uint64(f)) & np.uint64(1)) << np.uint64(kv - 1 - f)
-    return out
+    """Reversed kernel order: bit f of the result is bit kv - 1 - f of m.
+    ``__brev(kmask) >> (32 - kv)`` for one word (cumm/conv/kernel.py:883-889);
+    for dynamic masks the words are walked from the last with ``__brev`` and the
+    bit offset ``used * 32 - kv`` (mma.py:201-265), the same map."""
+    return _pack(mask_bits(m, kv)[:, ::-1], np.asarray(m))
+
+
+def _filter_bits(mask_filter: int, kv: int) -> np.ndarray:
+    """Offset f passes the filter when bit f % 32 of mask_filter is set
+    (kernel.py:891-897 with ``filter_offset % 32`` for dynamic masks)."""
+    return np.array([(mask_filter >> (f % 32)) & 1 for f in range(kv)], dtype=bool)
 
 
 def conv_forward(X, W, pair, mask_sorted, argsort, tile_m: int, mask_filter: int = ALL,
@@ -121,10 +150,10 @@ def conv_forward(X, W, pair, mask_sorted, argsort, tile_m: int, mask_filter: int
     n = argsort.shape[0]
     K, kv, C = W.shape
     tm = tile_masks(mask_sorted, n, tile_m, mask_filter)
-    row_mask = np.repeat(tm, tile_m)[:n]
+    row_bits = np.repeat(mask_bits(tm, kv), tile_m, axis=0)[:n]
     acc = np.zeros((n, K), dtype=np.float64)
     for f in range(kv):
-        rows = np.nonzero((row_mask >> np.uint64(f)) & np.uint64(1))[0]
+        rows = np.nonzero(row_bits[:, f])[0]
         if rows.size == 0:
             continue
         idx = pair[f][argsort[rows]]
@@ -156,13 +185,13 @@ def conv_dgrad(dO, W, pair_bwd, mask_sorted, argsort, tile_m: int, mask_filter: 
     n = argsort.shape[0]
     K, kv, C = W.shape
     raw = tile_masks(mask_sorted, n, tile_m, mask_filter)
-    live_t = raw != 0
+    live_t = mask_bits(raw, kv).any(axis=1)
     tm = _brev_kv(raw, kv) if reverse_mask else raw
-    row_mask = np.repeat(tm, tile_m)[:n]
+    row_bits = np.repeat(mask_bits(tm, kv), tile_m, axis=0)[:n]
     row_live = np.repeat(live_t, tile_m)[:n]
     acc = np.zeros((n, C), dtype=np.float64)
     for f in range(kv):
-        rows = np.nonzero(((row_mask >> np.uint64(f)) & np.uint64(1)).astype(bool) & row_live)[0]
+        rows = np.nonzero(row_bits[:, f] & row_live)[0]
         if rows.size == 0:
             continue
         idx = pair_bwd[f][argsort[rows]]
@@ -186,8 +215,8 @@ def conv_wgrad(X, dO, pair, mask_reduced, argsort, mask_width: int, mask_filter:
     """Wgrad (op 2): returns dW FP64 [K, kv, C].
 
     Offsets outside mask_filter are left untouched (kernel.py:891-897).  Sorted
-    row r contributes to offset f only if bit f is set in
-    ``mask_reduced[r // mask_width]`` (the forward mask_output; the k-tile skip
+    row r contributes to offset f only if offset f is set in
+    ``mask_reduced[r // mask_width]`` (one word, or words [W] with dynamic masks) (the forward mask_output; the k-tile skip
     of mma.py:607-648 with mask_width % TK == 0) and ``pair[f][argsort[r]] != -1``:
     ``dW[:, f, :] = epi(sum_r dO[argsort[r]]^T X[pair[f][argsort[r]]])``, the
     split-K partitions summing into one result (kernel.py:960-1083).
@@ -202,12 +231,14 @@ def conv_wgrad(X, dO, pair, mask_reduced, argsort, mask_width: int, mask_filter:
     r = np.arange(n)
     mi = r // mask_width
     in_range = mi < mask_reduced.shape[0]
-    rmask = np.zeros(n, dtype=np.uint64)
-    rmask[in_range] = mask_reduced[mi[in_range]].astype(np.uint64)
+    gbits = mask_bits(mask_reduced, kv)
+    rbits = np.zeros((n, kv), dtype=bool)
+    rbits[in_range] = gbits[mi[in_range]]
+    fbits = _filter_bits(mask_filter, kv)
     for f in range(kv):
-        if not (mask_filter >> f) & 1:
+        if not fbits[f]:
             continue
-        rows = np.nonzero((rmask >> np.uint64(f)) & np.uint64(1))[0]
+        rows = np.nonzero(rbits[:, f])[0]
         idx = pair[f][argsort[rows]]
         ok = idx >= 0
         acc = dO[argsort[rows[ok]]].T @ X[idx[ok]]

@@ -89,13 +89,13 @@ def conv_params(op: int, desp, inp: torch.Tensor, weight: torch.Tensor, out: tor
     return p
 
 
-def desp_for(op: int, dt: str, C: int, K: int, out_dt: Optional[str] = None):
+def desp_for(op: int, dt: str, C: int, K: int, out_dt: Optional[str] = None, dynamic: bool = False):
     o = DT[out_dt or dt]
     if op == 0:
-        return select_conv_algo(0, DT[dt], DT[dt], o, C, K)
+        return select_conv_algo(0, DT[dt], DT[dt], o, C, K, dynamic)
     if op == 1:
-        return select_conv_algo(1, o, DT[dt], DT[dt], C, K)
-    return select_conv_algo(2, DT[dt], o, DT[dt], C, K)
+        return select_conv_algo(1, o, DT[dt], DT[dt], C, K, dynamic)
+    return select_conv_algo(2, DT[dt], o, DT[dt], C, K, dynamic)
 
 
 def run_fwd(case: SubmCase, dt: str, out_dt: Optional[str] = None, **kw):

@@ -186,6 +186,57 @@ def test_int8_conv_equals_clipped_integer_bruteforce():
     assert (np.abs(bf) > 127).any() and (np.abs(bf) < 127).any()
 
 
+# --------------------------------------------------------------- dynamic (multi-word) masks
+def _tiny5(n=160, seed=4, C=4, K=3):
+    grid = (6, 16, 16)
+    coords = pc.plane_scene(n, grid=grid, seed=seed)
+    prob = pc.build_subm(coords, grid, ksize=(5, 5, 5))
+    rng = np.random.default_rng(seed + 3)
+    return prob, rng.standard_normal((n, C)), rng.standard_normal((K, 125, C)), rng.standard_normal((n, K))
+
+
+def test_dynamic_masks_5x5x5_match_bruteforce():
+    """kv = 125: masks are [N, 4] words; forward, dgrad (reversed masks) and
+    wgrad (reduced multi-word masks) equal the coordinate brute force."""
+    prob, X, W, dO = _tiny5()
+    assert prob.kv == 125 and prob.mask_sorted.shape == (prob.n, 4)
+    assert pc.mac_efficiency(prob.mask_sorted, 16, prob.num_pairs) <= 1.0
+    out, tm = ref.conv_forward(X, W, prob.pair_fwd, prob.mask_sorted, prob.argsort, 16)
+    assert np.allclose(out, bruteforce.conv_forward_coords(prob.coords, X, W), rtol=1e-12, atol=1e-12)
+    assert tm.shape == ((prob.n + 15) // 16, 4) and np.array_equal(tm, pc.tile_masks(prob.mask_sorted, 16))
+    din = ref.conv_dgrad(dO, W, prob.pair_bwd, prob.mask_sorted, prob.argsort, 16, reverse_mask=True)
+    assert np.allclose(din, bruteforce.conv_dgrad_coords(prob.coords, dO, W), rtol=1e-12, atol=1e-12)
+    dw = ref.conv_wgrad(X, dO, prob.pair_fwd, tm, prob.argsort, 16)
+    assert np.allclose(dw, bruteforce.conv_wgrad_coords(prob.coords, X, dO, 125), rtol=1e-12, atol=1e-12)
+
+
+def test_dynamic_mask_filter_is_per_bit_position():
+    """A filter bit b removes offsets b, b + 32, b + 64, b + 96 (filter_offset % 32)."""
+    prob, X, W, dO = _tiny5(seed=9)
+    filt = 0xFFFFFFFF & ~(1 << 5)
+    out, tm = ref.conv_forward(X, W, prob.pair_fwd, prob.mask_sorted, prob.argsort, 16, mask_filter=filt)
+    W0 = W.copy()
+    W0[:, [5, 37, 69, 101], :] = 0.0
+    assert np.allclose(out, bruteforce.conv_forward_coords(prob.coords, X, W0), rtol=1e-12, atol=1e-12)
+    assert not (tm & np.uint32(1 << 5)).any()
+    dw = ref.conv_wgrad(X, dO, prob.pair_fwd, tm, prob.argsort, 16, mask_filter=filt)
+    full = bruteforce.conv_wgrad_coords(prob.coords, X, dO, 125)
+    keep = [f for f in range(125) if f % 32 != 5]
+    assert np.allclose(dw[:, keep], full[:, keep]) and not dw[:, [5, 37, 69, 101]].any()
+
+
+def test_multiword_representation_of_small_kernels_is_equivalent():
+    """kv = 27 as [N, 1] words (a dynamic-mask layout) gives the 1-word results."""
+    prob, X, W, dO = _tiny(seed=13)
+    m2 = prob.mask_sorted.reshape(-1, 1)
+    a, ta = ref.conv_forward(X, W, prob.pair_fwd, prob.mask_sorted, prob.argsort, 16)
+    b, tb = ref.conv_forward(X, W, prob.pair_fwd, m2, prob.argsort, 16)
+    assert np.array_equal(a, b) and np.array_equal(ta, tb[:, 0])
+    a = ref.conv_dgrad(dO, W, prob.pair_bwd, prob.mask_sorted, prob.argsort, 16, reverse_mask=True)
+    b = ref.conv_dgrad(dO, W, prob.pair_bwd, m2, prob.argsort, 16, reverse_mask=True)
+    assert np.array_equal(a, b)
+
+
 # --------------------------------------------------------------- dgrad / wgrad
 def test_dgrad_adjoint_and_cpu_path():
     prob, X, W, dO = _tiny(seed=8)
