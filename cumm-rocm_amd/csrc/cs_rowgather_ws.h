@@ -3,16 +3,18 @@
 // Measured on MI355X, a dependent gather round trip under full load costs
 // ~1.5-2 us, and a kernel whose waves walk a tile's kernel offsets one round
 // trip at a time lasts as long as its heaviest tiles (up to 24 offsets).  This
-// kernel bounds every tile to three round trips (masks -> pair indices of up
-// to 8 offsets -> their gathered rows) and keeps register use low so 12 waves
-// fit on a CU:
+// kernel bounds every tile to three round trips (masks -> pair indices of a
+// batch of offsets -> their gathered rows) and keeps register use at or below
+// 128 VGPRs so 16 waves fit on a CU (more waves in flight measured faster than
+// deeper per-wave pipelines at 12 waves):
 //   * persistent blocks, one per CU: the weights of all kv offsets for a slice
 //     of NCOL output columns are loaded once into LDS (<= 108 KiB, 16-byte
 //     chunks XOR-swizzled by row so MFMA operand reads are conflict-free);
 //   * each wave takes 16-row mask tiles round-robin over the whole grid;
 //   * per tile: one load round trip for row masks / argsort, one for the pair
-//     indices of up to 8 offsets (each of the four lane groups loads a
-//     different offset), one for the gathered rows of those offsets (raw buffer
+//     indices of a batch of 8 / 4 / 2 offsets (1 / 2 / 4 reduction chunks;
+//     each lane group loads a different offset), one for the gathered rows of
+//     those offsets (raw buffer
 //     loads, index -1 reads zero); MFMAs take weights from LDS.
 // Used when kv * NCOL * C * sizeof(T) fits the LDS budget with NCOL >= 16.
 #pragma once
@@ -22,7 +24,7 @@
 namespace cs {
 
 constexpr int kWsLdsBytes = 110592;  // 108 KiB of weights per block
-constexpr int kWsWaves = 12;         // waves per block (3 per SIMD: 168 VGPRs each)
+constexpr int kWsWaves = 16;         // waves per block (4 per SIMD: <= 128 VGPRs each)
 constexpr int kWsStageBytes = 2048;  // per-wave LDS tile for coalesced row stores
 
 // KC: reduction chunks (C / chunk); NBC: 16-column blocks per wave (NCOL / 16)
@@ -35,7 +37,9 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
   constexpr int C = KC * CH;            // reduction length handled (== p.kred)
   constexpr int CHUNKS = C / E;         // 16-byte chunks per weight row
   constexpr int SWZ = CHUNKS >= 8 ? 7 : CHUNKS - 1;
-  constexpr int BATCH = KC > 2 ? 4 : 8;  // offsets per gather round trip
+  // offsets per gather round trip: 8 row chunks per lane in flight (measured on
+  // MI355X: 16 waves with these batches beat 12 waves with 8-offset batches)
+  constexpr int BATCH = KC == 1 ? 8 : (KC == 2 ? 4 : 2);
   __shared__ __attribute__((aligned(16))) char wlds[kWsLdsBytes];
   __shared__ __attribute__((aligned(16))) char stage_lds[kWsWaves * kWsStageBytes];
   __shared__ int tile_ctr;  // next entry of this block's tile pool
@@ -189,7 +193,8 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
     return bm ? __builtin_ctz(bm) : -1;
   };
   // next batch: the first BATCH offsets of rem (returned as a bit mask);
-  // pair indices issued (lane group lg loads slots lg and lg + 4), not waited on
+  // pair indices issued (lane group lg loads slot lg, and lg + 4 when BATCH is
+  // 8), not waited on
   auto batch = [&](uint32_t& rem, int col, int& ia, int& ib) -> uint32_t {
     uint32_t bm = 0u;
 #pragma unroll
@@ -201,7 +206,9 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
     int fs[BATCH];
 #pragma unroll
     for (int i = 0; i < BATCH; ++i) fs[i] = slot_f(bm, i);
-    const int fa = lg == 0 ? fs[0] : lg == 1 ? fs[1] : lg == 2 ? fs[2] : fs[3];
+    int fa;
+    if constexpr (BATCH >= 4) fa = lg == 0 ? fs[0] : lg == 1 ? fs[1] : lg == 2 ? fs[2] : fs[3];
+    else fa = lg == 0 ? fs[0] : lg == 1 ? fs[1 % BATCH] : -1;
     int fb = -1;
     if constexpr (BATCH == 8) fb = lg == 0 ? fs[4] : lg == 1 ? fs[5] : lg == 2 ? fs[6] : fs[7];
     if (p.pairs) {
@@ -409,7 +416,13 @@ static hipError_t launch_ws_kc(const RowGatherArgs& a, int ncol, hipStream_t s) 
   switch (ncol / 16) {
     case 1: hipLaunchKernelGGL((rowgather_ws_kernel<T, OutT, KC, 1>), grid, block, 0, s, a, ntiles); break;
     case 2: hipLaunchKernelGGL((rowgather_ws_kernel<T, OutT, KC, 2>), grid, block, 0, s, a, ntiles); break;
-    case 4: hipLaunchKernelGGL((rowgather_ws_kernel<T, OutT, KC, 4>), grid, block, 0, s, a, ntiles); break;
+    case 4:
+      // four reduction chunks with 64 columns do not fit 128 VGPRs; launch_ws caps them at 32
+      if constexpr (KC < 4) {
+        hipLaunchKernelGGL((rowgather_ws_kernel<T, OutT, KC, 4>), grid, block, 0, s, a, ntiles);
+        break;
+      }
+      return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -425,6 +438,7 @@ static hipError_t launch_ws(const RowGatherArgs& a, int max_ncol, hipStream_t s,
   if (ncol == 48) ncol = 32;
   if (ncol == 0 || a.pair_col == nullptr) return hipSuccess;
   const int kc = a.kred / Mma<T>::kChunk;
+  if (kc == 4 && ncol > 32) ncol = 32;
   *used = true;
   switch (kc) {
     case 1: return launch_ws_kc<T, OutT, 1>(a, ncol, s);
