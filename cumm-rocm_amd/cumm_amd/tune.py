@@ -62,6 +62,9 @@ def candidates(p: ConvParams) -> List[ConvAlgoDesp]:
             continue
         if (d.dtype_input, d.dtype_weight, d.dtype_output) != (d0.dtype_input, d0.dtype_weight, d0.dtype_output):
             continue
+        # same mask kind and int8 form as the caller's desp (the masks' layout depends on it)
+        if bool(d.dynamic_mask) != bool(d0.dynamic_mask) or bool(d.is_int8_inference) != bool(d0.is_int8_inference):
+            continue
         m, n, k = ConvMainUnitTest.extract_mnk(int(d.op_type), rows, C, K, kv, -1, -1, True)
         if not d.supported(m, n, k, C, K, p.mask_width):
             continue
