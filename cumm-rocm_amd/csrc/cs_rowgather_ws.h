@@ -26,6 +26,7 @@ namespace cs {
 constexpr int kWsLdsBytes = 110592;  // 108 KiB of weights per block
 constexpr int kWsWaves = 16;         // waves per block (4 per SIMD: <= 128 VGPRs each)
 constexpr int kWsStageBytes = 2048;  // per-wave LDS tile for coalesced row stores
+constexpr int kWsPoolMax = 4096;     // tiles per block ordered by cost (larger pools keep grid order)
 
 // KC: reduction chunks (C / chunk); NBC: 16-column blocks per wave (NCOL / 16)
 template <typename T, typename OutT, int KC, int NBC>
@@ -43,6 +44,9 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
   __shared__ __attribute__((aligned(16))) char wlds[kWsLdsBytes];
   __shared__ __attribute__((aligned(16))) char stage_lds[kWsWaves * kWsStageBytes];
   __shared__ int tile_ctr;  // next entry of this block's tile pool
+  __shared__ uint16_t pool[kWsPoolMax];  // pool entries, heaviest tiles first
+  __shared__ uint8_t pcost[kWsPoolMax];  // offsets per pool entry
+  __shared__ int hist[34];               // pool entries per cost bucket (32 - offsets)
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -66,89 +70,39 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
     }
   };
 
-  // Each wave's first two tiles are fixed (pool entries wave and
-  // kWsWaves + wave); their row data loads together with the weights.
-  const int t0 = (int)std::min<int64_t>((int64_t)blockIdx.x + (int64_t)gridDim.x * wave, ntiles);
-  const int t1 = (int)std::min<int64_t>((int64_t)blockIdx.x + (int64_t)gridDim.x * (kWsWaves + wave), ntiles);
-  Meta Mc, Mn;
-
-  // ---- weights of all offsets for columns [n0, n0 + NCOL) -> LDS.  Each
-  // thread issues its (at most WPT) loads before writing any of them, so the
-  // preload costs one round trip instead of one per chunk.
-  if (p.b_kmajor) {
-    // B element (f, n, k) at b[k * b_ks + f * b_fs + n] (dgrad reads the KRSC
-    // weights directly): a unit of E reduction rows x E columns loads as E
-    // 16-byte rows and is transposed in registers into E chunks along k.
-    constexpr int S = (int)sizeof(T);
-    constexpr int NU = NCOL / E, KU = C / E;
-    constexpr int UPT = (32 * NU * KU + 64 * kWsWaves - 1) / (64 * kWsWaves);  // kv <= 32
-    const T* B = reinterpret_cast<const T*>(p.b);
-    const int units = p.kv * NU * KU;
-    u32x4 v[UPT][E];
-#pragma unroll
-    for (int j = 0; j < UPT; ++j) {
-      const int u = tid + j * (64 * kWsWaves);
-      const int f = u / (NU * KU), nu = (u / KU) % NU, ku = u % KU;
-      const bool ok = u < units && n0 + nu * E < p.nout;
-#pragma unroll
-      for (int i = 0; i < E; ++i)
-        v[j][i] = ok ? ld16(B + (int64_t)(ku * E + i) * p.b_ks + (int64_t)f * p.b_fs + n0 + nu * E) : zero16();
-    }
-    load_meta(t0, Mc);
-    load_meta(t1, Mn);
-#pragma unroll
-    for (int j = 0; j < UPT; ++j) {
-      const int u = tid + j * (64 * kWsWaves);
-      if (u < units) {
-        const int f = u / (NU * KU), nu = (u / KU) % NU, ku = u % KU;
-#pragma unroll
-        for (int jn = 0; jn < E; ++jn) {
-          u32x4 o = zero16();
-#pragma unroll
-          for (int i = 0; i < E; ++i) {
-            const uint32_t w = v[j][i][(jn * S) >> 2];
-            uint32_t e;
-            if constexpr (S == 4) {
-              e = w;
-            } else {
-              e = (w >> (((jn * S) & 3) * 8)) & ((1u << (8 * S)) - 1u);
-            }
-            o[(i * S) >> 2] |= e << (((i * S) & 3) * 8);
-          }
-          const int n = nu * E + jn;
-          *reinterpret_cast<u32x4*>(wlds + ((f * NCOL + n) * CHUNKS + (ku ^ (n & SWZ))) * 16) = o;
-        }
-      }
-    }
-  } else {
-    const T* B = reinterpret_cast<const T*>(p.b);
-    const int total = p.kv * NCOL * CHUNKS;
-    constexpr int WPT = (kWsLdsBytes / 16 + 64 * kWsWaves - 1) / (64 * kWsWaves);
-    u32x4 v[WPT];
-#pragma unroll
-    for (int j = 0; j < WPT; ++j) {
-      const int i = tid + j * (64 * kWsWaves);
-      const int c = i % CHUNKS;
-      const int n = (i / CHUNKS) % NCOL;
-      const int f = i / (CHUNKS * NCOL);
-      const bool ok = i < total && n0 + n < p.nout;
-      v[j] = ok ? ld16(B + (int64_t)f * p.b_fs + (int64_t)(n0 + n) * p.b_ns + c * E) : zero16();
-    }
-    load_meta(t0, Mc);
-    load_meta(t1, Mn);
-#pragma unroll
-    for (int j = 0; j < WPT; ++j) {
-      const int i = tid + j * (64 * kWsWaves);
-      if (i < total) {
-        const int c = i % CHUNKS;
-        const int n = (i / CHUNKS) % NCOL;
-        const int f = i / (CHUNKS * NCOL);
-        *reinterpret_cast<u32x4*>(wlds + ((f * NCOL + n) * CHUNKS + (c ^ (n & SWZ))) * 16) = v[j];
-      }
+  // Block b's pool is tiles b, b + G, b + 2G, ... (G = gridDim.x: a uniform
+  // sample of the tile order, so pools weigh about the same).  A tile costs
+  // about one gather round trip per batch of its offsets, and a wave runs only
+  // a few tiles, so the pool is ordered heaviest first (longest-processing-
+  // time order) before the waves draw it: entry k's cost is the popcount of
+  // its tile mask.  Which wave runs a tile does not change any result.
+  const int P = blockIdx.x < ntiles ? (ntiles - 1 - (int)blockIdx.x) / (int)gridDim.x + 1 : 0;
+  const bool ordered = P <= kWsPoolMax;
+  auto pool_tile = [&](int v) -> int {  // tile of pool position v
+    if (v >= P) return ntiles;
+    const int k = ordered ? (int)pool[v] : v;
+    return (int)blockIdx.x + (int)gridDim.x * k;
+  };
+  // row data of the first 64 pool entries (16 lanes each), issued before the
+  // weights: it orders the pool and is the first tiles' row data
+  constexpr int kWsFirst = 64 * kWsWaves / kMaskTile;  // pool entries of the first pass
+  static_assert(sizeof(Meta) * 64 * kWsWaves <= kWsWaves * kWsStageBytes, "first-pass row data exceeds the stage buffers");
+  Meta M0;
+  M0.m = 0u;
+  M0.col = 0;
+  M0.orow = 0;
+  if (ordered) {
+    const int k = tid >> 4;
+    const int64_t r = ((int64_t)blockIdx.x + (int64_t)gridDim.x * k) * kMaskTile + (tid & 15);
+    if (k < P && r < p.m) {
+      M0.m = p.mask ? p.mask[r] : kvmask;
+      M0.col = p.pair_col ? p.pair_col[r] : (int)r;
+      M0.orow = p.out_row ? p.out_row[r] : (int)r;
     }
   }
-  if (tid == 0) tile_ctr = 2 * kWsWaves;
-  __syncthreads();
+  int t0 = ntiles, t1 = ntiles;
+  Meta Mc, Mn;
+
 
   // per-column epilogue constants of this lane's columns, loaded once
   float ep_s[NBC], ep_b[NBC];
@@ -325,24 +279,154 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
   // The wave's tiles are software-pipelined: while the gathered rows of tile
   // k are in flight, the pair indices of tile k+1 and the row data of tile
   // k+2 load too, so a tile costs about one round trip.  Tiles with more than
-  // BATCH offsets finish their extra batches inline.
-  // Block b's pool is tiles b, b + gridDim.x, b + 2 gridDim.x, ... (a uniform
-  // sample of the tile order, so pools weigh about the same); its waves take
-  // the pool's entries in turn through an LDS counter, so a wave that drew
-  // heavy tiles takes fewer.  Results do not depend on which wave runs a tile.
+  // BATCH offsets run their further batches with the next batch's pair indices
+  // in flight.  The waves take the ordered pool's entries in turn through an
+  // LDS counter, so a wave that drew heavy tiles takes fewer.
   auto next_tile = [&]() -> int {
     int v = 0;
     if (lane == 0) v = atomicAdd(&tile_ctr, 1);
     v = __builtin_amdgcn_readfirstlane(v);
-    const int64_t k = (int64_t)blockIdx.x + (int64_t)gridDim.x * v;
-    return k < ntiles ? (int)k : ntiles;
+    return pool_tile(v);
   };
+  // ---- weights of all offsets for columns [n0, n0 + NCOL) -> LDS.  Each
+  // thread issues its (at most WPT) loads before writing any of them, so the
+  // preload costs one round trip instead of one per chunk.  The first tile's
+  // row data loads before the weights and its pair indices are issued while
+  // the weights are in flight (loads complete in issue order, so the LDS
+  // stores wait for the weights only).
+  bool live_c = false;
+  uint32_t rem_c = 0u, bm_c = 0u;
+  int ia_c = -1, ib_c = -1;
+  auto first_tile = [&]() {
+    // order the pool: counting sort by offsets, descending
+    if (ordered) {
+      if (tid < 34) hist[tid] = 0;
+      __syncthreads();
+      // the first pass's row data, in the (free until the first epilogue) stage buffers
+      Meta* sm = reinterpret_cast<Meta*>(stage_lds);
+      sm[tid] = M0;
+      for (int k0 = 0; k0 < P; k0 += kWsFirst) {
+        const int k = k0 + (tid >> 4);
+        uint32_t m = M0.m;
+        if (k0 > 0) {
+          m = 0u;
+          const int64_t r = ((int64_t)blockIdx.x + (int64_t)gridDim.x * k) * kMaskTile + (tid & 15);
+          if (k < P && r < p.m) m = p.mask ? p.mask[r] : kvmask;
+        }
+        m |= __shfl_xor(m, 1);
+        m |= __shfl_xor(m, 2);
+        m |= __shfl_xor(m, 4);
+        m |= __shfl_xor(m, 8);
+        if ((tid & 15) == 0 && k < P) {
+          const int c = __builtin_popcount(m & p.mask_filter & kvmask);
+          pcost[k] = (uint8_t)c;
+          atomicAdd(&hist[32 - c], 1);
+        }
+      }
+      __syncthreads();
+      if (tid == 0) {
+        int acc = 0;
+        for (int i = 0; i < 33; ++i) {
+          const int h = hist[i];
+          hist[i] = acc;
+          acc += h;
+        }
+      }
+      __syncthreads();
+      for (int k = tid; k < P; k += 64 * kWsWaves) pool[atomicAdd(&hist[32 - pcost[k]], 1)] = (uint16_t)k;
+      __syncthreads();
+    }
+    // each wave's first two tiles: pool positions wave and kWsWaves + wave;
+    // row data of first-pass entries comes from LDS
+    t0 = pool_tile(wave);
+    t1 = pool_tile(kWsWaves + wave);
+    auto first_meta = [&](int v, int t, Meta& M) {
+      const int k = ordered && v < P ? (int)pool[v] : kWsFirst;
+      if (k < kWsFirst) M = reinterpret_cast<const Meta*>(stage_lds)[k * kMaskTile + lr];
+      else load_meta(t, M);
+    };
+    first_meta(wave, t0, Mc);
+    first_meta(kWsWaves + wave, t1, Mn);
+    __syncthreads();  // the stage buffers are free again
+    if (t0 < ntiles) {
+      rem_c = prepare(tile_mask(t0, Mc.m), live_c);
+      bm_c = batch(rem_c, Mc.col, ia_c, ib_c);
+    }
+  };
+  if (p.b_kmajor) {
+    // B element (f, n, k) at b[k * b_ks + f * b_fs + n] (dgrad reads the KRSC
+    // weights directly): a unit of E reduction rows x E columns loads as E
+    // 16-byte rows and is transposed in registers into E chunks along k.
+    constexpr int S = (int)sizeof(T);
+    constexpr int NU = NCOL / E, KU = C / E;
+    constexpr int UPT = (32 * NU * KU + 64 * kWsWaves - 1) / (64 * kWsWaves);  // kv <= 32
+    const T* B = reinterpret_cast<const T*>(p.b);
+    const int units = p.kv * NU * KU;
+    u32x4 v[UPT][E];
+#pragma unroll
+    for (int j = 0; j < UPT; ++j) {
+      const int u = tid + j * (64 * kWsWaves);
+      const int f = u / (NU * KU), nu = (u / KU) % NU, ku = u % KU;
+      const bool ok = u < units && n0 + nu * E < p.nout;
+#pragma unroll
+      for (int i = 0; i < E; ++i)
+        v[j][i] = ok ? ld16(B + (int64_t)(ku * E + i) * p.b_ks + (int64_t)f * p.b_fs + n0 + nu * E) : zero16();
+    }
+    first_tile();
+#pragma unroll
+    for (int j = 0; j < UPT; ++j) {
+      const int u = tid + j * (64 * kWsWaves);
+      if (u < units) {
+        const int f = u / (NU * KU), nu = (u / KU) % NU, ku = u % KU;
+#pragma unroll
+        for (int jn = 0; jn < E; ++jn) {
+          u32x4 o = zero16();
+#pragma unroll
+          for (int i = 0; i < E; ++i) {
+            const uint32_t w = v[j][i][(jn * S) >> 2];
+            uint32_t e;
+            if constexpr (S == 4) {
+              e = w;
+            } else {
+              e = (w >> (((jn * S) & 3) * 8)) & ((1u << (8 * S)) - 1u);
+            }
+            o[(i * S) >> 2] |= e << (((i * S) & 3) * 8);
+          }
+          const int n = nu * E + jn;
+          *reinterpret_cast<u32x4*>(wlds + ((f * NCOL + n) * CHUNKS + (ku ^ (n & SWZ))) * 16) = o;
+        }
+      }
+    }
+  } else {
+    const T* B = reinterpret_cast<const T*>(p.b);
+    const int total = p.kv * NCOL * CHUNKS;
+    constexpr int WPT = (kWsLdsBytes / 16 + 64 * kWsWaves - 1) / (64 * kWsWaves);
+    u32x4 v[WPT];
+#pragma unroll
+    for (int j = 0; j < WPT; ++j) {
+      const int i = tid + j * (64 * kWsWaves);
+      const int c = i % CHUNKS;
+      const int n = (i / CHUNKS) % NCOL;
+      const int f = i / (CHUNKS * NCOL);
+      const bool ok = i < total && n0 + n < p.nout;
+      v[j] = ok ? ld16(B + (int64_t)f * p.b_fs + (int64_t)(n0 + n) * p.b_ns + c * E) : zero16();
+    }
+    first_tile();
+#pragma unroll
+    for (int j = 0; j < WPT; ++j) {
+      const int i = tid + j * (64 * kWsWaves);
+      if (i < total) {
+        const int c = i % CHUNKS;
+        const int n = (i / CHUNKS) % NCOL;
+        const int f = i / (CHUNKS * NCOL);
+        *reinterpret_cast<u32x4*>(wlds + ((f * NCOL + n) * CHUNKS + (c ^ (n & SWZ))) * 16) = v[j];
+      }
+    }
+  }
+  if (tid == 0) tile_ctr = 2 * kWsWaves;
+  __syncthreads();
   int t = t0;
   if (t >= ntiles) return;
-  bool live_c;
-  uint32_t rem_c = prepare(tile_mask(t, Mc.m), live_c);
-  int ia_c, ib_c;
-  uint32_t bm_c = batch(rem_c, Mc.col, ia_c, ib_c);
   int tn = t1;
   while (true) {
     u32x4 af[BATCH][KC];
@@ -358,15 +442,25 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
       bm_n = batch(rem_n, Mn.col, ia_n, ib_n);
       load_meta(tnn, Mnn);
     }
+    // a tile with more than BATCH offsets: the pair indices of each further
+    // batch load while the rows of the batch before it are gathered, so every
+    // further batch costs one round trip
+    uint32_t bm_x = 0u;
+    int ia_x = -1, ib_x = -1;
+    if (rem_c) bm_x = batch(rem_c, Mc.col, ia_x, ib_x);
     f32x4 acc[NBC];
 #pragma unroll
     for (int nb = 0; nb < NBC; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
     mma_batch(bm_c, af, acc);
-    while (rem_c) {
-      int ia, ib;
-      const uint32_t bm = batch(rem_c, Mc.col, ia, ib);
-      gather(bm, ia, ib, af);
-      mma_batch(bm, af, acc);
+    while (bm_x) {
+      gather(bm_x, ia_x, ib_x, af);
+      uint32_t bm_y = 0u;
+      int ia_y = -1, ib_y = -1;
+      if (rem_c) bm_y = batch(rem_c, Mc.col, ia_y, ib_y);
+      mma_batch(bm_x, af, acc);
+      bm_x = bm_y;
+      ia_x = ia_y;
+      ib_x = ib_y;
     }
     if (live_c) epilogue(t, Mc.orow, acc);
     if (tn >= ntiles) break;
