@@ -173,6 +173,8 @@ class Layer:
         self.names = {k: repr(v.conv_algo_desp) for k, v in self.params.items()}
         self.run = ConvMainUnitTest.implicit_gemm2
         self.ops = [k for k in ("fwd", "dgrad", "wgrad") if k in ops]
+        if "wgrad" in self.ops and "fwd" not in self.ops:
+            self.run(self.params["fwd"])  # wgrad reads the forward's per-tile masks
 
     def step(self):
         for k in self.ops:

@@ -5,10 +5,10 @@
 // wgrad_*          : sparse-conv wgrad / ShuffleAB GEMM.  Per offset, a pair
 //                     list (a row, b row) in original row order is built from
 //                     the pair table and the effective row masks; the main
-//                     kernel stages 64 gathered rows of both operands into LDS
-//                     transposed (channel-major), so MFMA operands are
-//                     contiguous along the reduction (row) dimension, and
-//                     writes an FP32 partial per piece of the list.
+//                     kernel stages gathered rows of both operands pair-major
+//                     in LDS, reads MFMA operands along the pair dimension
+//                     with LDS transpose reads, and writes an FP32 partial per
+//                     piece of the list.
 // splitk_reduce     : fixed pairwise tree over the split partials + epilogue.
 #include <climits>
 #include <cstdint>
@@ -19,30 +19,66 @@
 namespace cs {
 
 // ============================================================ wgrad
-template <typename T> struct TrStore;
-template <> struct TrStore<float> {
-  __device__ __forceinline__ static void put(char* lds, int pitch, int ch, int row, u32x4 v) {
-#pragma unroll
-    for (int e = 0; e < 4; ++e)
-      *reinterpret_cast<uint32_t*>(lds + (ch + e) * pitch + row * 4) = v[e];
+// Stage tiles are pair-major in LDS: row = pair, the tile's channels
+// contiguous (pitch P bytes), filled by coalesced 16-byte row loads.  Mma<T>
+// wants each lane's operand elements along the pair (reduction) dimension:
+// lane (i = lane & 15, g = lane >> 4) of the fragment at channel c0 holds
+// T[k0 + g*E + e][c0 + i].  gfx950's LDS transpose reads deliver exactly that
+// from pair-major rows: within each 16-lane group, ds_read_b64_tr_b16 gives
+// lane i element e = the (i % 4)-th value read by lane (i / 4) + 4e, and
+// ds_read_b64_tr_b8 gives lane i byte e = the (i % 8)-th byte read by lane
+// (i / 8) + 2e; so lane j points at pair row k0 + g*E + j/4 (j/2 for bytes),
+// channels c0 + 4*(j % 4) (c0 + 8*(j % 2)).
+typedef short wg_v4i16 __attribute__((ext_vector_type(4)));
+typedef int wg_v2i32 __attribute__((ext_vector_type(2)));
+#define WG_LDS __attribute__((address_space(3)))
+
+template <typename T> struct WgFrag;
+template <> struct WgFrag<float> {
+  static constexpr int kPad = 16;
+  // 16x16x4 f32: element e of lane (i, g) feeds MFMA e at reduction index 4g + e
+  __device__ __forceinline__ static u32x4 load(const char* lds, int pitch, int k0, int c0, int lane) {
+    const char* q = lds + (k0 + 4 * (lane >> 4)) * pitch + (c0 + (lane & 15)) * 4;
+    return u32x4{*reinterpret_cast<const uint32_t*>(q), *reinterpret_cast<const uint32_t*>(q + pitch),
+                 *reinterpret_cast<const uint32_t*>(q + 2 * pitch), *reinterpret_cast<const uint32_t*>(q + 3 * pitch)};
   }
 };
-template <> struct TrStore<_Float16> {
-  __device__ __forceinline__ static void put(char* lds, int pitch, int ch, int row, u32x4 v) {
-#pragma unroll
-    for (int e = 0; e < 8; ++e)
-      *reinterpret_cast<uint16_t*>(lds + (ch + e) * pitch + row * 2) =
-          (uint16_t)((v[e >> 1] >> (16 * (e & 1))) & 0xffffu);
+template <> struct WgFrag<_Float16> {
+  static constexpr int kPad = 32;
+  __device__ __forceinline__ static u32x4 load(const char* lds, int pitch, int k0, int c0, int lane) {
+    const int j = lane & 15, g = lane >> 4;
+    const char* q = lds + (k0 + 8 * g + (j >> 2)) * pitch + (c0 + 4 * (j & 3)) * 2;
+    const wg_v4i16 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((WG_LDS wg_v4i16*)(q));
+    const wg_v4i16 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((WG_LDS wg_v4i16*)(q + 4 * pitch));
+    const wg_v2i32 l2 = __builtin_bit_cast(wg_v2i32, lo), h2 = __builtin_bit_cast(wg_v2i32, hi);
+    return u32x4{(uint32_t)l2.x, (uint32_t)l2.y, (uint32_t)h2.x, (uint32_t)h2.y};
   }
 };
-template <> struct TrStore<bf16> : TrStore<_Float16> {};
-template <> struct TrStore<e4m3> {
-  __device__ __forceinline__ static void put(char* lds, int pitch, int ch, int row, u32x4 v) {
-#pragma unroll
-    for (int e = 0; e < 16; ++e)
-      *reinterpret_cast<uint8_t*>(lds + (ch + e) * pitch + row) =
-          (uint8_t)((v[e >> 2] >> (8 * (e & 3))) & 0xffu);
+template <> struct WgFrag<bf16> : WgFrag<_Float16> {};
+template <> struct WgFrag<e4m3> {
+  static constexpr int kPad = 16;
+  __device__ __forceinline__ static u32x4 load(const char* lds, int pitch, int k0, int c0, int lane) {
+    const int j = lane & 15, g = lane >> 4;
+    const char* q = lds + (k0 + 16 * g + (j >> 1)) * pitch + c0 + 8 * (j & 1);
+    const wg_v2i32 lo = __builtin_amdgcn_ds_read_tr8_b64_v2i32((WG_LDS wg_v2i32*)(q));
+    const wg_v2i32 hi = __builtin_amdgcn_ds_read_tr8_b64_v2i32((WG_LDS wg_v2i32*)(q + 8 * pitch));
+    return u32x4{(uint32_t)lo.x, (uint32_t)lo.y, (uint32_t)hi.x, (uint32_t)hi.y};
   }
+};
+
+// Stage geometry of the wgrad kernel for a TM x TN tile: R pairs per stage
+// (about 16 KB of rows, a multiple of the MFMA reduction chunk), CPA / CPB
+// 16-byte chunks per A / B row, LA / LB chunks per thread and stage.
+template <typename T, int TM, int TN>
+struct WgGeom {
+  static constexpr int CH = Mma<T>::kChunk;
+  static constexpr int R0 = 16384 / ((TM + TN) * (int)sizeof(T));
+  static constexpr int R = R0 < CH ? CH : (R0 > 128 ? 128 : R0 / CH * CH);
+  static constexpr int CPA = TM * (int)sizeof(T) / 16, CPB = TN * (int)sizeof(T) / 16;
+  static constexpr int LA = R * CPA / 256, LB = R * CPB / 256;
+  static constexpr int PA = TM * (int)sizeof(T) + WgFrag<T>::kPad, PB = TN * (int)sizeof(T) + WgFrag<T>::kPad;
+  static_assert(R * CPA % 256 == 0 && R * CPB % 256 == 0, "stage rows must fill the block's loads");
+  static_assert(R % CH == 0, "stage must hold whole MFMA chunks");
 };
 
 // pieces of offset f's pair list: nb = min(max_blocks, div_up(count, min_rows))
@@ -168,30 +204,21 @@ __global__ __launch_bounds__(256) void wgrad_list_kernel(WgradArgs p) {
   if (blk == nblk - 1 && tid == 0) p.total[f] = base;
 }
 
-// Rows staged per thread and chunk: 4 (256-row chunks) for 1- and 2-byte types,
-// 2 for f32, whose 16-byte loads cover half as many channels.
-template <typename T> struct WgRows { static constexpr int value = 4; };
-template <> struct WgRows<float> { static constexpr int value = 2; };
-
 // Partial products over one piece of offset f's pair list: rows a[list_a] and
-// b[list_b], 64 x 64 output tile, FP32 partial per piece.
-// TM x TN output tile per block (64 or 128 each): a wider tile gathers every
-// row once for more output channels.  Four waves in a 2 x 2 grid.
+// b[list_b], TM x TN output tile (64 or 128 each), FP32 partial per piece.
+// Four waves in a 2 x 2 grid.  Every 16-byte load covers a contiguous piece
+// of one gathered row (CPA threads per row), and two stages of rows are in
+// flight while the MFMAs consume the stage in LDS.
 template <typename T, int TM, int TN>
-__global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs p) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void wgrad_kernel(WgradArgs p) {
   using MM = Mma<T>;
-  constexpr int E = MM::kElems;
-  constexpr int CH = MM::kChunk;
-  constexpr int R = kWgradRows;
-  constexpr int PITCH = R * (int)sizeof(T) + 16;
-  constexpr int CPA = (TM / E + 3) / 4;   // A chunks per thread and row (TM channels)
-  constexpr int CPB = (TN / E + 3) / 4;   // B chunks per thread and row
-  // rows per thread and chunk: fewer when the tile is wide, to bound registers
-  constexpr int RPT = (TM + TN > 128) ? (sizeof(T) == 4 ? 1 : 2) : WgRows<T>::value;
-  constexpr int CROWS = RPT * R;          // rows per chunk
+  using G = WgGeom<T, TM, TN>;
+  constexpr int R = G::R, CH = G::CH, LA = G::LA, LB = G::LB, CPA = G::CPA, CPB = G::CPB;
+  constexpr int PA = G::PA, PB = G::PB;
+  constexpr int E = 16 / (int)sizeof(T);     // channels per 16-byte chunk
   constexpr int FA = TM / 32, FB = TN / 32;  // 16-wide fragments per wave
-  __shared__ __attribute__((aligned(16))) char lds_a[TM * PITCH];
-  __shared__ __attribute__((aligned(16))) char lds_b[TN * PITCH];
+  __shared__ __attribute__((aligned(16))) char lds_a[R * PA];
+  __shared__ __attribute__((aligned(16))) char lds_b[R * PB];
 
   const int f = blockIdx.y;
   if (!((p.mask_filter >> (f & 31)) & 1u)) return;
@@ -204,13 +231,14 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs p) {
   const int nn0 = (blockIdx.x % tiles_n) * TN;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const int lr = lane & 15, lg = lane >> 4;
   const int wm = (wave & 1) * (TM / 2), wn = (wave >> 1) * (TN / 2);
-  const int srow = tid & 63, sq = tid >> 6;
   const int per = (count + nb - 1) / nb;
   const int q0 = piece * per;
   const int nrows_blk = min(per, count - q0);
-  const int nchunks = (nrows_blk + CROWS - 1) / CROWS;
+  const int nst = (nrows_blk + R - 1) / R;
+  // this thread's chunk of every A / B row it loads (256 is a multiple of CPA, CPB)
+  const int ca = tid % CPA, cb = tid % CPB;
+  const bool oka = m0 + ca * E < p.M, okb = nn0 + cb * E < p.N;
 
   f32x4 acc[FA][FB];
 #pragma unroll
@@ -225,75 +253,81 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs p) {
   const __amdgpu_buffer_rsrc_t rLa = make_rsrc(p.list_a + lofs, (uint32_t)nrows_blk * 4u);
   const __amdgpu_buffer_rsrc_t rLb = make_rsrc(p.list_b + lofs, (uint32_t)nrows_blk * 4u);
 
-  // Two-stage pipeline: the list entries of chunk j+1 load with the rows of chunk j.
-  auto ld_idx = [&](int j, int (&ra)[RPT], int (&rb)[RPT]) {
+  auto ld_idx = [&](int s, int (&ia)[LA], int (&ib)[LB]) {
 #pragma unroll
-    for (int st = 0; st < RPT; ++st) {
-      const int pos = j * CROWS + st * R + srow;
-      const uint32_t off = pos < nrows_blk ? (uint32_t)pos * 4u : kBufOOB;
-      ra[st] = (int)__builtin_amdgcn_raw_buffer_load_b32(rLa, off, 0, 0);
-      rb[st] = (int)__builtin_amdgcn_raw_buffer_load_b32(rLb, off, 0, 0);
+    for (int l = 0; l < LA; ++l) {
+      const int pos = s * R + (l * 256 + tid) / CPA;
+      ia[l] = (int)__builtin_amdgcn_raw_buffer_load_b32(rLa, pos < nrows_blk ? (uint32_t)pos * 4u : kBufOOB, 0, 0);
+    }
+#pragma unroll
+    for (int l = 0; l < LB; ++l) {
+      const int pos = s * R + (l * 256 + tid) / CPB;
+      ib[l] = (int)__builtin_amdgcn_raw_buffer_load_b32(rLb, pos < nrows_blk ? (uint32_t)pos * 4u : kBufOOB, 0, 0);
     }
   };
-  int ra0[RPT], rb0[RPT];
-  ld_idx(0, ra0, rb0);
-  for (int j = 0; j < nchunks; ++j) {
-    u32x4 va[RPT][CPA], vb[RPT][CPB];
+  // rows past the piece's end load zeros (out-of-range offsets)
+  auto ld_rows = [&](int s, const int (&ia)[LA], const int (&ib)[LB], u32x4 (&va)[LA], u32x4 (&vb)[LB]) {
 #pragma unroll
-    for (int st = 0; st < RPT; ++st) {
-      const bool ok = j * CROWS + st * R + srow < nrows_blk;
-#pragma unroll
-      for (int i = 0; i < CPA; ++i) {
-        const int ch = (sq + 4 * i) * E;
-        const bool oka = ok && ch < TM && m0 + ch < p.M;
-        va[st][i] = bld16(rA, oka ? (uint32_t)(((int64_t)ra0[st] * p.lda + m0 + ch) * (int)sizeof(T)) : kBufOOB);
-      }
-#pragma unroll
-      for (int i = 0; i < CPB; ++i) {
-        const int ch = (sq + 4 * i) * E;
-        const bool okb = ok && ch < TN && nn0 + ch < p.N;
-        vb[st][i] = bld16(rB, okb ? (uint32_t)(((int64_t)rb0[st] * p.ldb + nn0 + ch) * (int)sizeof(T)) : kBufOOB);
-      }
-    }
-    int ra1[RPT], rb1[RPT];
-    ld_idx(j + 1, ra1, rb1);
-#pragma unroll
-    for (int st = 0; st < RPT; ++st) {
-      __syncthreads();  // previous stage's MFMAs are done with the tiles
-#pragma unroll
-      for (int i = 0; i < CPA; ++i) {
-        const int ch = (sq + 4 * i) * E;
-        if (ch < TM) TrStore<T>::put(lds_a, PITCH, ch, srow, va[st][i]);
-      }
-#pragma unroll
-      for (int i = 0; i < CPB; ++i) {
-        const int ch = (sq + 4 * i) * E;
-        if (ch < TN) TrStore<T>::put(lds_b, PITCH, ch, srow, vb[st][i]);
-      }
-      __syncthreads();
-#pragma unroll
-      for (int kc = 0; kc < R; kc += CH) {
-        u32x4 af[FA], bfr[FB];
-#pragma unroll
-        for (int i = 0; i < FA; ++i)
-          af[i] = *reinterpret_cast<const u32x4*>(lds_a + (wm + 16 * i + lr) * PITCH + (kc + lg * E) * (int)sizeof(T));
-#pragma unroll
-        for (int jj = 0; jj < FB; ++jj)
-          bfr[jj] = *reinterpret_cast<const u32x4*>(lds_b + (wn + 16 * jj + lr) * PITCH + (kc + lg * E) * (int)sizeof(T));
-#pragma unroll
-        for (int i = 0; i < FA; ++i)
-#pragma unroll
-          for (int jj = 0; jj < FB; ++jj) MM::step(acc[i][jj], af[i], bfr[jj]);
-      }
+    for (int l = 0; l < LA; ++l) {
+      const bool ok = oka && s * R + (l * 256 + tid) / CPA < nrows_blk;
+      va[l] = bld16(rA, ok ? (uint32_t)(((int64_t)ia[l] * p.lda + m0 + ca * E) * (int)sizeof(T)) : kBufOOB);
     }
 #pragma unroll
-    for (int st = 0; st < RPT; ++st) {
-      ra0[st] = ra1[st];
-      rb0[st] = rb1[st];
+    for (int l = 0; l < LB; ++l) {
+      const bool ok = okb && s * R + (l * 256 + tid) / CPB < nrows_blk;
+      vb[l] = bld16(rB, ok ? (uint32_t)(((int64_t)ib[l] * p.ldb + nn0 + cb * E) * (int)sizeof(T)) : kBufOOB);
     }
+  };
+  auto st_rows = [&](const u32x4 (&va)[LA], const u32x4 (&vb)[LB]) {
+#pragma unroll
+    for (int l = 0; l < LA; ++l)
+      *reinterpret_cast<u32x4*>(lds_a + ((l * 256 + tid) / CPA) * PA + ca * 16) = va[l];
+#pragma unroll
+    for (int l = 0; l < LB; ++l)
+      *reinterpret_cast<u32x4*>(lds_b + ((l * 256 + tid) / CPB) * PB + cb * 16) = vb[l];
+  };
+  auto compute = [&]() {
+#pragma unroll
+    for (int kc = 0; kc < R; kc += CH) {
+      u32x4 bfr[FB];
+#pragma unroll
+      for (int j = 0; j < FB; ++j) bfr[j] = WgFrag<T>::load(lds_b, PB, kc, wn + 16 * j, lane);
+#pragma unroll
+      for (int i = 0; i < FA; ++i) {
+        const u32x4 af = WgFrag<T>::load(lds_a, PA, kc, wm + 16 * i, lane);
+#pragma unroll
+        for (int j = 0; j < FB; ++j) MM::step(acc[i][j], af, bfr[j]);
+      }
+    }
+  };
+
+  // stage s's rows sit in (va0, vb0) for even s and (va1, vb1) for odd s;
+  // the stage after next is requested while the MFMAs consume stage s.
+  int ia0[LA], ib0[LB], ia1[LA], ib1[LB];
+  u32x4 va0[LA], vb0[LB], va1[LA], vb1[LB];
+  ld_idx(0, ia0, ib0);
+  ld_idx(1, ia1, ib1);
+  ld_rows(0, ia0, ib0, va0, vb0);
+  ld_rows(1, ia1, ib1, va1, vb1);
+  ld_idx(2, ia0, ib0);
+  for (int s = 0; s < nst; s += 2) {
+    __syncthreads();  // the previous stage's MFMAs are done with the tiles
+    st_rows(va0, vb0);
+    ld_rows(s + 2, ia0, ib0, va0, vb0);
+    ld_idx(s + 3, ia1, ib1);
+    __syncthreads();
+    compute();
+    if (s + 1 >= nst) break;
+    __syncthreads();
+    st_rows(va1, vb1);
+    ld_rows(s + 3, ia1, ib1, va1, vb1);
+    ld_idx(s + 4, ia0, ib0);
+    __syncthreads();
+    compute();
   }
 
   float* out = p.partial + ((int64_t)f * p.max_blocks + piece) * (int64_t)p.M * p.N;
+  const int lr = lane & 15, lg = lane >> 4;
 #pragma unroll
   for (int i = 0; i < FA; ++i)
 #pragma unroll
