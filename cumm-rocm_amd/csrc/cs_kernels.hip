@@ -168,11 +168,14 @@ __global__ __launch_bounds__(256) void wgrad_list_kernel(WgradArgs p) {
     if (blk == 0 && tid == 0) p.total[f] = 0;
     return;
   }
+  // base = valid pairs of this offset in earlier blocks; the first 256 counts
+  // load as one unconditional buffer load issued with the row loads, so both
+  // round trips overlap
+  const __amdgpu_buffer_rsrc_t rc = make_rsrc(p.bcount + f * nblk, (uint32_t)blk * 4u);
+  int part = (int)__builtin_amdgcn_raw_buffer_load_b32(rc, (uint32_t)tid * 4u, 0, 0);
   int jv[16];
   const uint32_t bits = wgrad_rows(p, f, i0, tid, jv);
-  // base = valid pairs of this offset in earlier blocks
-  int part = 0;
-  for (int b = tid; b < blk; b += 256) part += p.bcount[f * nblk + b];
+  for (int b = tid + 256; b < blk; b += 256) part += p.bcount[f * nblk + b];
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) part += __shfl_xor(part, d);
   // per-wave counts of every 256-row step k
