@@ -131,6 +131,17 @@ template <> struct Mma<int8_t> {
   }
 };
 
+// Slot of 16-byte chunk c of row n in an LDS tile of CHUNKS chunks per row
+// (CHUNKS a power of two) read one chunk per row by 16 consecutive rows at a
+// time (MFMA operand reads): slot = c ^ chunk_swz<CHUNKS>(n).  LDS has 64
+// banks of 4 bytes, so 16 / CHUNKS rows share one bank span; the swizzle gives
+// those rows distinct slots and the 16 reads land on distinct banks.
+template <int CHUNKS> __device__ __forceinline__ int chunk_swz(int n) {
+  constexpr int kRowsPerSpan = CHUNKS >= 16 ? 1 : 16 / CHUNKS;
+  constexpr int kMask = (CHUNKS < 16 ? CHUNKS : 16) - 1;
+  return (n / kRowsPerSpan) & kMask;
+}
+
 template <typename T> struct IsInt8 { static constexpr bool value = false; };
 template <> struct IsInt8<int8_t> { static constexpr bool value = true; };
 

@@ -22,13 +22,22 @@ namespace cs {
 // Stage tiles are pair-major in LDS: row = pair, the tile's channels
 // contiguous (pitch P bytes), filled by coalesced 16-byte row loads.  Mma<T>
 // wants each lane's operand elements along the pair (reduction) dimension:
-// lane (i = lane & 15, g = lane >> 4) of the fragment at channel c0 holds
-// T[k0 + g*E + e][c0 + i].  gfx950's LDS transpose reads deliver exactly that
-// from pair-major rows: within each 16-lane group, ds_read_b64_tr_b16 gives
-// lane i element e = the (i % 4)-th value read by lane (i / 4) + 4e, and
-// ds_read_b64_tr_b8 gives lane i byte e = the (i % 8)-th byte read by lane
-// (i / 8) + 2e; so lane j points at pair row k0 + g*E + j/4 (j/2 for bytes),
-// channels c0 + 4*(j % 4) (c0 + 8*(j % 2)).
+// lane (i = lane & 15, g = lane >> 4) of the fragment at channel c0 holds E
+// elements T[k(g, e)][c0 + i], where k(g, e) is any fixed one-to-one map of
+// (lane group, element) onto the chunk's pairs (A and B use the same map).
+// gfx950's LDS transpose reads deliver this from pair-major rows: within each
+// 16-lane group, ds_read_b64_tr_b16 gives lane i element e = the (i % 4)-th
+// value read by lane (i / 4) + 4e, and ds_read_b64_tr_b8 gives lane i byte e =
+// the (i % 8)-th byte read by lane (i / 8) + 2e; so lane j points at pair row
+// kb + j/4 (j/2 for bytes), channels c0 + 4*(j % 4) (c0 + 8*(j % 2)).
+//
+// LDS has 64 banks of 4 bytes.  16-bit and 8-bit tiles are unpadded and their
+// 16-byte chunks sit at slot chunk ^ swz(row): with the row maps below the 32
+// lanes that the LDS serves together (two lane groups) read rows kb .. kb + 7
+// (16-bit) or kb .. kb + 15 (8-bit) of one chunk pair or chunk, and swz puts
+// them on distinct banks; row stores (16 lanes together) stay conflict-free.
+// FP32 tiles keep 16 bytes of row padding (their 16x16x4 reads are plain
+// 4-byte loads from four rows 4 apart, conflict-free at that pitch).
 typedef short wg_v4i16 __attribute__((ext_vector_type(4)));
 typedef int wg_v2i32 __attribute__((ext_vector_type(2)));
 #define WG_LDS __attribute__((address_space(3)))
@@ -36,7 +45,9 @@ typedef int wg_v2i32 __attribute__((ext_vector_type(2)));
 template <typename T> struct WgFrag;
 template <> struct WgFrag<float> {
   static constexpr int kPad = 16;
+  template <int CPR> __device__ __forceinline__ static int swz(int) { return 0; }
   // 16x16x4 f32: element e of lane (i, g) feeds MFMA e at reduction index 4g + e
+  template <int CPR>
   __device__ __forceinline__ static u32x4 load(const char* lds, int pitch, int k0, int c0, int lane) {
     const char* q = lds + (k0 + 4 * (lane >> 4)) * pitch + (c0 + (lane & 15)) * 4;
     return u32x4{*reinterpret_cast<const uint32_t*>(q), *reinterpret_cast<const uint32_t*>(q + pitch),
@@ -44,24 +55,40 @@ template <> struct WgFrag<float> {
   }
 };
 template <> struct WgFrag<_Float16> {
-  static constexpr int kPad = 32;
+  static constexpr int kPad = 0;
+  // CPR = 8 or 16 chunks per row
+  template <int CPR> __device__ __forceinline__ static int swz(int r) { return (r & 7) * (CPR / 8); }
+  // elements 0-3 of lane group g: pairs k0 + 4g + e; elements 4-7: k0 + 16 + 4g + e
+  template <int CPR>
   __device__ __forceinline__ static u32x4 load(const char* lds, int pitch, int k0, int c0, int lane) {
     const int j = lane & 15, g = lane >> 4;
-    const char* q = lds + (k0 + 8 * g + (j >> 2)) * pitch + (c0 + 4 * (j & 3)) * 2;
-    const wg_v4i16 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((WG_LDS wg_v4i16*)(q));
-    const wg_v4i16 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((WG_LDS wg_v4i16*)(q + 4 * pitch));
+    const int ch = c0 + 4 * (j & 3);
+    const int r0 = k0 + 4 * g + (j >> 2), r1 = r0 + 16;
+    const char* q0 = lds + r0 * pitch + (((ch >> 3) ^ swz<CPR>(r0)) << 4) + (ch & 7) * 2;
+    const char* q1 = lds + r1 * pitch + (((ch >> 3) ^ swz<CPR>(r1)) << 4) + (ch & 7) * 2;
+    const wg_v4i16 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((WG_LDS wg_v4i16*)(q0));
+    const wg_v4i16 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((WG_LDS wg_v4i16*)(q1));
     const wg_v2i32 l2 = __builtin_bit_cast(wg_v2i32, lo), h2 = __builtin_bit_cast(wg_v2i32, hi);
     return u32x4{(uint32_t)l2.x, (uint32_t)l2.y, (uint32_t)h2.x, (uint32_t)h2.y};
   }
 };
 template <> struct WgFrag<bf16> : WgFrag<_Float16> {};
 template <> struct WgFrag<e4m3> {
-  static constexpr int kPad = 16;
+  static constexpr int kPad = 0;
+  // CPR = 4 or 8 chunks per row
+  template <int CPR> __device__ __forceinline__ static int swz(int r) {
+    return CPR == 4 ? (r >> 2) & 3 : (r >> 1) & 7;
+  }
+  // bytes 0-7 of lane group g: pairs k0 + 8g + e; bytes 8-15: k0 + 32 + 8g + e
+  template <int CPR>
   __device__ __forceinline__ static u32x4 load(const char* lds, int pitch, int k0, int c0, int lane) {
     const int j = lane & 15, g = lane >> 4;
-    const char* q = lds + (k0 + 16 * g + (j >> 1)) * pitch + c0 + 8 * (j & 1);
-    const wg_v2i32 lo = __builtin_amdgcn_ds_read_tr8_b64_v2i32((WG_LDS wg_v2i32*)(q));
-    const wg_v2i32 hi = __builtin_amdgcn_ds_read_tr8_b64_v2i32((WG_LDS wg_v2i32*)(q + 8 * pitch));
+    const int ch = c0 + 8 * (j & 1);
+    const int r0 = k0 + 8 * g + (j >> 1), r1 = r0 + 32;
+    const char* q0 = lds + r0 * pitch + (((ch >> 4) ^ swz<CPR>(r0)) << 4) + (ch & 15);
+    const char* q1 = lds + r1 * pitch + (((ch >> 4) ^ swz<CPR>(r1)) << 4) + (ch & 15);
+    const wg_v2i32 lo = __builtin_amdgcn_ds_read_tr8_b64_v2i32((WG_LDS wg_v2i32*)(q0));
+    const wg_v2i32 hi = __builtin_amdgcn_ds_read_tr8_b64_v2i32((WG_LDS wg_v2i32*)(q1));
     return u32x4{(uint32_t)lo.x, (uint32_t)lo.y, (uint32_t)hi.x, (uint32_t)hi.y};
   }
 };
@@ -283,21 +310,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void w
   };
   auto st_rows = [&](const u32x4 (&va)[LA], const u32x4 (&vb)[LB]) {
 #pragma unroll
-    for (int l = 0; l < LA; ++l)
-      *reinterpret_cast<u32x4*>(lds_a + ((l * 256 + tid) / CPA) * PA + ca * 16) = va[l];
+    for (int l = 0; l < LA; ++l) {
+      const int r = (l * 256 + tid) / CPA;
+      *reinterpret_cast<u32x4*>(lds_a + r * PA + ((ca ^ WgFrag<T>::template swz<CPA>(r)) << 4)) = va[l];
+    }
 #pragma unroll
-    for (int l = 0; l < LB; ++l)
-      *reinterpret_cast<u32x4*>(lds_b + ((l * 256 + tid) / CPB) * PB + cb * 16) = vb[l];
+    for (int l = 0; l < LB; ++l) {
+      const int r = (l * 256 + tid) / CPB;
+      *reinterpret_cast<u32x4*>(lds_b + r * PB + ((cb ^ WgFrag<T>::template swz<CPB>(r)) << 4)) = vb[l];
+    }
   };
   auto compute = [&]() {
 #pragma unroll
     for (int kc = 0; kc < R; kc += CH) {
       u32x4 bfr[FB];
 #pragma unroll
-      for (int j = 0; j < FB; ++j) bfr[j] = WgFrag<T>::load(lds_b, PB, kc, wn + 16 * j, lane);
+      for (int j = 0; j < FB; ++j) bfr[j] = WgFrag<T>::template load<CPB>(lds_b, PB, kc, wn + 16 * j, lane);
 #pragma unroll
       for (int i = 0; i < FA; ++i) {
-        const u32x4 af = WgFrag<T>::load(lds_a, PA, kc, wm + 16 * i, lane);
+        const u32x4 af = WgFrag<T>::template load<CPA>(lds_a, PA, kc, wm + 16 * i, lane);
 #pragma unroll
         for (int j = 0; j < FB; ++j) MM::step(acc[i][j], af, bfr[j]);
       }

@@ -33,7 +33,7 @@ __global__ __launch_bounds__(64 * kStWaves) void rowgather_stream_kernel(RowGath
   constexpr int C = KC * CH;              // reduction length (== p.kred)
   constexpr int NCOL = NB * 16;
   constexpr int CHUNKS = C / E;           // 16-byte chunks per weight row
-  constexpr int SWZ = CHUNKS >= 8 ? 7 : CHUNKS - 1;
+  static_assert((CHUNKS & (CHUNKS - 1)) == 0, "chunk swizzle needs a power-of-two row");
   constexpr int STAGE = NCOL * CHUNKS * 16;  // bytes of one offset's weights
   static_assert(2 * STAGE <= kStLdsBytes, "weight stage too large");
   constexpr int NT = 64 * kStWaves;
@@ -113,7 +113,7 @@ __global__ __launch_bounds__(64 * kStWaves) void rowgather_stream_kernel(RowGath
       const int i = tid + j * NT;
       if (i < NCOL * CHUNKS) {
         const int n = i / CHUNKS, c = i % CHUNKS;
-        *reinterpret_cast<u32x4*>(base + (n * CHUNKS + (c ^ (n & SWZ))) * 16) = wv[j];
+        *reinterpret_cast<u32x4*>(base + (n * CHUNKS + (c ^ chunk_swz<CHUNKS>(n))) * 16) = wv[j];
       }
     }
   };
@@ -165,7 +165,7 @@ __global__ __launch_bounds__(64 * kStWaves) void rowgather_stream_kernel(RowGath
           for (int nb = 0; nb < NB; ++nb) {
             const int n = nb * 16 + lr;
             const int c = kc * (CH / E) + lg;
-            const u32x4 bfr = *reinterpret_cast<const u32x4*>(wb + (n * CHUNKS + (c ^ (n & SWZ))) * 16);
+            const u32x4 bfr = *reinterpret_cast<const u32x4*>(wb + (n * CHUNKS + (c ^ chunk_swz<CHUNKS>(n))) * 16);
             MM::step(acc[nb], a_cur[kc], bfr);
           }
         }

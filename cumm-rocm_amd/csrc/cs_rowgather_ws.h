@@ -37,7 +37,7 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
   constexpr int NCOL = NBC * 16;
   constexpr int C = KC * CH;            // reduction length handled (== p.kred)
   constexpr int CHUNKS = C / E;         // 16-byte chunks per weight row
-  constexpr int SWZ = CHUNKS >= 8 ? 7 : CHUNKS - 1;
+  static_assert((CHUNKS & (CHUNKS - 1)) == 0, "chunk swizzle needs a power-of-two row");
   // offsets per gather round trip: 8 row chunks per lane in flight (measured on
   // MI355X: 16 waves with these batches beat 12 waves with 8-offset batches)
   constexpr int BATCH = KC == 1 ? 8 : (KC == 2 ? 4 : 2);
@@ -203,7 +203,7 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
           const int n = nb * 16 + lr;
           const int c = kc * (CH / E) + lg;
           const u32x4 bfr =
-              *reinterpret_cast<const u32x4*>(wlds + ((f * NCOL + n) * CHUNKS + (c ^ (n & SWZ))) * 16);
+              *reinterpret_cast<const u32x4*>(wlds + ((f * NCOL + n) * CHUNKS + (c ^ chunk_swz<CHUNKS>(n))) * 16);
           MM::step(acc[nb], af[i][kc], bfr);
         }
       }
@@ -393,7 +393,7 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
             o[(i * S) >> 2] |= e << (((i * S) & 3) * 8);
           }
           const int n = nu * E + jn;
-          *reinterpret_cast<u32x4*>(wlds + ((f * NCOL + n) * CHUNKS + (ku ^ (n & SWZ))) * 16) = o;
+          *reinterpret_cast<u32x4*>(wlds + ((f * NCOL + n) * CHUNKS + (ku ^ chunk_swz<CHUNKS>(n))) * 16) = o;
         }
       }
     }
@@ -419,7 +419,7 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
         const int c = i % CHUNKS;
         const int n = (i / CHUNKS) % NCOL;
         const int f = i / (CHUNKS * NCOL);
-        *reinterpret_cast<u32x4*>(wlds + ((f * NCOL + n) * CHUNKS + (c ^ (n & SWZ))) * 16) = v[j];
+        *reinterpret_cast<u32x4*>(wlds + ((f * NCOL + n) * CHUNKS + (c ^ chunk_swz<CHUNKS>(n))) * 16) = v[j];
       }
     }
   }
