@@ -111,7 +111,7 @@ __global__ void subm_lookup(const int32_t* __restrict__ coords, int64_t n, Shape
 }
 
 // Strided (stride 2, padding 1): thread (f, o) looks up o*2 - 1 + k_f among
-// the inputs; the inverse entry goes to pair_bwd[f][i].
+// the inputs; the inverse entry goes to pair_bwd[f][i] (when pair_bwd is set).
 __global__ void strided_lookup(const int32_t* __restrict__ out_coords, int64_t n_out, int64_t n_in, Shape3 in_s,
                                const uint64_t* tkeys, const int32_t* tvals, uint64_t tmask, int32_t* pair_fwd,
                                int32_t* pair_bwd) {
@@ -125,7 +125,7 @@ __global__ void strided_lookup(const int32_t* __restrict__ out_coords, int64_t n
   if (z >= 0 && z < in_s.s0 && y >= 0 && y < in_s.s1 && x >= 0 && x < in_s.s2)
     i = hash_find(key_of(c[0], z, y, x, in_s), tkeys, tvals, tmask);
   pair_fwd[(int64_t)f * n_out + o] = i;
-  if (i >= 0 && i < n_in) pair_bwd[(int64_t)f * n_in + i] = (int32_t)o;
+  if (pair_bwd != nullptr && i >= 0 && i < n_in) pair_bwd[(int64_t)f * n_in + i] = (int32_t)o;
 }
 
 __global__ void row_masks(const int32_t* __restrict__ pair, int64_t n, uint32_t* mask) {
@@ -330,8 +330,10 @@ int strided_impl(const int32_t* in_coords, int64_t n_in, const int32_t* in_shape
     hipLaunchKernelGGL(hash_insert, dim3((unsigned)((n_in + 255) / 256)), dim3(256), 0, s, in_coords, n_in, is,
                        tkeys, tvals, cap - 1);
     PCHK(hipGetLastError());
-    hipLaunchKernelGGL(fill_i32, grid1(kKv * n_in), dim3(256), 0, s, pair_bwd, kKv * n_in, -1);
-    PCHK(hipGetLastError());
+    if (pair_bwd != nullptr) {
+      hipLaunchKernelGGL(fill_i32, grid1(kKv * n_in), dim3(256), 0, s, pair_bwd, kKv * n_in, -1);
+      PCHK(hipGetLastError());
+    }
   }
   if (n_out > 0) {
     hipLaunchKernelGGL(strided_lookup, dim3((unsigned)((n_out + 255) / 256), kKv), dim3(256), 0, s, out_coords,
@@ -340,7 +342,7 @@ int strided_impl(const int32_t* in_coords, int64_t n_in, const int32_t* in_shape
   }
   const int64_t mark = cv.off;
   int r = sort_masks(pair_fwd, n_out, mfs, afs, cv, s);
-  if (r) return r;
+  if (r || pair_bwd == nullptr) return r;  // forward-only tables (inference)
   cv.off = mark;  // the backward sort reuses the scratch; both are ordered on one stream
   return sort_masks(pair_bwd, n_in, mbs, abs_, cv, s);
 }
@@ -415,8 +417,12 @@ int cs_pairs_strided(const int32_t* in_coords, int64_t n_in, const int32_t* in_s
   cs_pairs_strided_workspace(n_in, n_out, &need);
   if (n_in > 0x7fffffff / 27 || n_out > 0x7fffffff / 27 || cs::bad_shape(in_shape))
     return cs::fail("bad size or spatial shape");
+  // the backward side (pair_bwd, mask_bwd_sorted, argsort_bwd) is all set or all null
+  const bool bwd = pair_bwd != nullptr;
+  if (bwd != (mask_bwd_sorted != nullptr) || bwd != (argsort_bwd != nullptr))
+    return cs::fail("backward tables must be all set or all null");
   if (ws_bytes < need || !workspace || (n_out > 0 && (!out_coords || !pair_fwd || !mask_fwd_sorted || !argsort_fwd)) ||
-      (n_in > 0 && (!in_coords || !pair_bwd || !mask_bwd_sorted || !argsort_bwd)))
+      (n_in > 0 && !in_coords))
     return cs::fail("null buffer or workspace too small");
   cs::Carver cv{static_cast<char*>(workspace)};
   return cs::strided_impl(in_coords, n_in, in_shape, out_coords, n_out, pair_fwd, pair_bwd, mask_fwd_sorted,

@@ -120,7 +120,9 @@ class DeviceLayer:
         kv = t.pair_fwd.shape[0]
         self.kv = kv
         self.pf = _dev_idx(t.pair_fwd, dev)
-        self.pb = _dev_idx(t.pair_bwd, dev)
+        if train and t.pair_bwd is None:
+            raise ValueError("training needs backward pair tables (build them with backward=True)")
+        self.pb = _dev_idx(t.pair_bwd, dev) if t.pair_bwd is not None else None
         self.mf = _dev_idx(t.mask_fwd, dev)
         self.af = _dev_idx(t.argsort_fwd, dev)
         self.mb = _dev_idx(t.mask_bwd, dev) if t.mask_bwd is not None else None

@@ -29,7 +29,7 @@ class DeviceTables:
     in_shape: Tuple[int, int, int]
     out_shape: Tuple[int, int, int]
     pair_fwd: torch.Tensor         # [27, N_out] int32
-    pair_bwd: torch.Tensor         # [27, N_in] int32
+    pair_bwd: Optional[torch.Tensor]  # [27, N_in] int32 (None: forward-only tables)
     mask_fwd: torch.Tensor         # [N_out] int32 view of uint32 masks, sorted
     argsort_fwd: torch.Tensor
     mask_bwd: Optional[torch.Tensor]
@@ -59,11 +59,18 @@ def _shape_ptr(shape) -> C.Array:
     return (C.c_int32 * 3)(*[int(v) for v in shape])
 
 
+def _ptr(t: Optional[torch.Tensor]):
+    return t.data_ptr() if t is not None else None
+
+
 def _ws(nbytes: int, dev) -> torch.Tensor:
     return torch.empty((max(int(nbytes), 1),), dtype=torch.uint8, device=dev)
 
 
-def subm_tables(coords: torch.Tensor, shape: Sequence[int], stream: Optional[int] = None) -> DeviceTables:
+def subm_tables(coords: torch.Tensor, shape: Sequence[int], stream: Optional[int] = None,
+                backward: bool = True) -> DeviceTables:
+    """``backward=False`` builds forward-only tables (no pair_bwd), all that
+    inference needs."""
     L = _abi.lib()
     coords = coords.contiguous()
     assert coords.dtype == torch.int32 and coords.dim() == 2 and coords.shape[1] == 4 and coords.is_cuda
@@ -71,13 +78,13 @@ def subm_tables(coords: torch.Tensor, shape: Sequence[int], stream: Optional[int
     dev = coords.device
     s = torch.cuda.current_stream().cuda_stream if stream is None else stream
     pf = torch.empty((27, n), dtype=torch.int32, device=dev)
-    pb = torch.empty((27, n), dtype=torch.int32, device=dev)
+    pb = torch.empty((27, n), dtype=torch.int32, device=dev) if backward else None
     ms = torch.empty((n,), dtype=torch.int32, device=dev)
     am = torch.empty((n,), dtype=torch.int32, device=dev)
     nb = C.c_int64(0)
     _check(L.cs_pairs_subm_workspace(n, C.byref(nb)))
     ws = _ws(nb.value, dev)
-    _check(L.cs_pairs_subm(coords.data_ptr(), n, _shape_ptr(shape), pf.data_ptr(), pb.data_ptr(), ms.data_ptr(),
+    _check(L.cs_pairs_subm(coords.data_ptr(), n, _shape_ptr(shape), pf.data_ptr(), _ptr(pb), ms.data_ptr(),
                            am.data_ptr(), ws.data_ptr(), ws.numel(), s))
     shp = tuple(int(v) for v in shape)
     return DeviceTables("subm", coords, coords, shp, shp, pf, pb, ms, am, None, None)
@@ -104,7 +111,8 @@ def strided_out_coords(coords: torch.Tensor, shape: Sequence[int], rule: str = "
 
 
 def strided_tables(coords: torch.Tensor, shape: Sequence[int], rule: str = "down",
-                   stream: Optional[int] = None) -> DeviceTables:
+                   stream: Optional[int] = None, backward: bool = True) -> DeviceTables:
+    """``backward=False`` skips pair_bwd and the backward masks / argsort."""
     L = _abi.lib()
     coords = coords.contiguous()
     dev = coords.device
@@ -112,25 +120,26 @@ def strided_tables(coords: torch.Tensor, shape: Sequence[int], rule: str = "down
     oc, out_shape = strided_out_coords(coords, shape, rule, s)
     n_in, n_out = coords.shape[0], oc.shape[0]
     pf = torch.empty((27, n_out), dtype=torch.int32, device=dev)
-    pb = torch.empty((27, n_in), dtype=torch.int32, device=dev)
+    pb = torch.empty((27, n_in), dtype=torch.int32, device=dev) if backward else None
     mf = torch.empty((n_out,), dtype=torch.int32, device=dev)
     af = torch.empty((n_out,), dtype=torch.int32, device=dev)
-    mb = torch.empty((n_in,), dtype=torch.int32, device=dev)
-    ab = torch.empty((n_in,), dtype=torch.int32, device=dev)
+    mb = torch.empty((n_in,), dtype=torch.int32, device=dev) if backward else None
+    ab = torch.empty((n_in,), dtype=torch.int32, device=dev) if backward else None
     nb = C.c_int64(0)
     _check(L.cs_pairs_strided_workspace(n_in, n_out, C.byref(nb)))
     ws = _ws(nb.value, dev)
     _check(L.cs_pairs_strided(coords.data_ptr(), n_in, _shape_ptr(shape), oc.data_ptr(), n_out, pf.data_ptr(),
-                              pb.data_ptr(), mf.data_ptr(), af.data_ptr(), mb.data_ptr(), ab.data_ptr(),
+                              _ptr(pb), mf.data_ptr(), af.data_ptr(), _ptr(mb), _ptr(ab),
                               ws.data_ptr(), ws.numel(), s))
     return DeviceTables("strided", coords, oc, tuple(int(v) for v in shape), out_shape, pf, pb, mf, af, mb, ab)
 
 
 def stack_tables(kinds: Sequence[str], coords: torch.Tensor, shape: Sequence[int], rule: str = "down",
-                 stream: Optional[int] = None):
+                 stream: Optional[int] = None, backward: bool = True):
     out = []
     for k in kinds:
-        t = subm_tables(coords, shape, stream) if k == "subm" else strided_tables(coords, shape, rule, stream)
+        t = (subm_tables(coords, shape, stream, backward) if k == "subm"
+             else strided_tables(coords, shape, rule, stream, backward))
         out.append(t)
         coords, shape = t.out_coords, t.out_shape
     return out

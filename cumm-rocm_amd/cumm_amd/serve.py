@@ -145,7 +145,7 @@ class SparseConvServer:
         with torch.cuda.stream(s):
             c_dev = pc_[:coords.shape[0]].to("cuda", non_blocking=True)
             x_dev = px_[:feats.shape[0]].to("cuda", non_blocking=True).to(B.TORCH_DT[self.dt])
-            tabs = pairs.stack_tables(self.kinds, c_dev, self.grid, stream=s.cuda_stream)
+            tabs = pairs.stack_tables(self.kinds, c_dev, self.grid, stream=s.cuda_stream, backward=False)
             st = B.ConvStack(tabs, self.channels, self.dt, self.weights, s.cuda_stream, train=False)
             st.bind(x_dev)
             st.step()

@@ -171,7 +171,9 @@ int cs_pairs_strided_out(const int32_t* coords, int64_t n, const int32_t* in_sha
                          uint64_t stream);
 /* Strided pair tables: pair_fwd [27][n_out] (input rows), pair_bwd [27][n_in]
  * (output rows), forward masks over the outputs and backward masks over the
- * inputs, each sorted with its argsort. */
+ * inputs, each sorted with its argsort.  Forward-only callers (inference)
+ * pass pair_bwd, mask_bwd_sorted and argsort_bwd all NULL; cs_pairs_subm
+ * likewise accepts pair_bwd = NULL. */
 int cs_pairs_strided_workspace(int64_t n_in, int64_t n_out, int64_t* nbytes);
 int cs_pairs_strided(const int32_t* in_coords, int64_t n_in, const int32_t* in_shape, const int32_t* out_coords,
                      int64_t n_out, int32_t* pair_fwd, int32_t* pair_bwd, uint32_t* mask_fwd_sorted,

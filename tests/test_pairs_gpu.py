@@ -88,3 +88,19 @@ def test_device_tables_drive_a_stack():
         outs.append([st.layers[-1].out.clone(), st.layers[1].din.clone(), st.layers[0].dw.clone()])
     for a, b in zip(*outs):
         assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("kind", ["subm", "strided"])
+def test_forward_only_tables(kind):
+    """backward=False (inference) gives the same forward tables and no backward ones."""
+    print(f"[pairs] forward-only {kind}", flush=True)
+    grid = (16, 96, 96)
+    c = torch.from_numpy(_coords([4000, 700], grid, 21)).cuda()
+    build = pairs.subm_tables if kind == "subm" else pairs.strided_tables
+    full, fwd = build(c, grid), build(c, grid, backward=False)
+    torch.cuda.synchronize()
+    for name in ("out_coords", "pair_fwd", "mask_fwd", "argsort_fwd"):
+        assert torch.equal(getattr(full, name), getattr(fwd, name)), name
+    assert fwd.pair_bwd is None and fwd.mask_bwd is None and fwd.argsort_bwd is None
+    with pytest.raises(ValueError, match="backward"):
+        B.DeviceLayer(fwd, 16, 16, "f16", np.zeros((16, 27, 16), np.float32), 0)
