@@ -385,7 +385,10 @@ template <typename OutT, int V>
 __global__ __launch_bounds__(256) void splitk_reduce_kernel(ReduceArgs p) {
   constexpr int PQ = 16;                     // piece lanes per element
   constexpr int PER = kWgradMaxBlocks / PQ;  // pieces per thread, at most
-  __shared__ float red[PQ][16 * V + 1];
+  // red[q][c * 16 + el]: element el * V + c of piece lane q.  The pitch puts
+  // the four piece lanes of one wave on distinct banks (64 banks).
+  constexpr int RP = V == 4 ? 16 * V + 16 : 16 * V;
+  __shared__ float red[PQ][RP];
   const int64_t mn = (int64_t)p.M * p.N;
   const int tid = threadIdx.x;
   const int el = tid & 15, q = tid >> 4;
@@ -427,7 +430,7 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(ReduceArgs p) {
     acc[0] = v[0];
   }
 #pragma unroll
-  for (int c = 0; c < V; ++c) red[q][el * V + c] = acc[c];
+  for (int c = 0; c < V; ++c) red[q][c * 16 + el] = acc[c];
   __syncthreads();
   // 16 * V elements, 256 threads: thread tid finishes element tid (if any)
   if (tid >= 16 * V) return;
@@ -437,7 +440,7 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(ReduceArgs p) {
   if (!((p.mask_filter >> (fo & 31)) & 1u)) return;
   float t[PQ];
 #pragma unroll
-  for (int i = 0; i < PQ; ++i) t[i] = red[i][tid];
+  for (int i = 0; i < PQ; ++i) t[i] = red[i][(tid % V) * 16 + tid / V];
 #pragma unroll
   for (int w = PQ / 2; w >= 1; w >>= 1)
 #pragma unroll
