@@ -189,8 +189,6 @@ __global__ __launch_bounds__(256) void wgrad_list_kernel(WgradArgs p) {
   const int nblk = gridDim.x;
   const int i0 = blk * kWgradListRows;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  __shared__ int wsum[16][4];
-  __shared__ int sbase;
   if (!((p.mask_filter >> (f & 31)) & 1u)) {
     if (blk == 0 && tid == 0) p.total[f] = 0;
     return;
@@ -205,33 +203,43 @@ __global__ __launch_bounds__(256) void wgrad_list_kernel(WgradArgs p) {
   for (int b = tid + 256; b < blk; b += 256) part += p.bcount[f * nblk + b];
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) part += __shfl_xor(part, d);
-  // per-wave counts of every 256-row step k
+  // per-wave counts of every 256-row step k; wpre[k][w] becomes the position
+  // of wave w's first pair of step k within the block (steps in order, waves
+  // in order inside a step): one 64-lane scan instead of per-entry sums
+  __shared__ int wpre[16][4];
+  __shared__ int wpart[4];
+  uint32_t balc[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     const uint64_t bal = __ballot((bits >> k) & 1u);
-    if (lane == 0) wsum[k][wave] = __popcll(bal);
+    balc[k] = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) wpre[k][wave] = __popcll(bal);
   }
-  __shared__ int wpart[4];
   if (lane == 0) wpart[wave] = part;
   __syncthreads();
-  if (tid == 0) sbase = wpart[0] + wpart[1] + wpart[2] + wpart[3];
+  if (tid < 64) {
+    const int c = wpre[tid >> 2][tid & 3];
+    int incl = c;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int t = __shfl_up(incl, d);
+      if (tid >= d) incl += t;
+    }
+    const int base0 = wpart[0] + wpart[1] + wpart[2] + wpart[3];
+    wpre[tid >> 2][tid & 3] = base0 + incl - c;
+    if (tid == 63 && blk == nblk - 1) p.total[f] = base0 + incl;
+  }
   __syncthreads();
-  int base = sbase;
   int32_t* la = p.list_a + (int64_t)f * p.list_ld;
   int32_t* lb = p.list_b + (int64_t)f * p.list_ld;
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
-    const bool v = (bits >> k) & 1u;
-    const uint64_t bal = __ballot(v);
-    int off = base + __popcll(bal & ((1ull << lane) - 1ull));
-    for (int w = 0; w < wave; ++w) off += wsum[k][w];
-    if (v) {
+    if ((bits >> k) & 1u) {
+      const int off = wpre[k][wave] + (int)balc[k];
       la[off] = i0 + tid + 256 * k;
       lb[off] = jv[k];
     }
-    base += wsum[k][0] + wsum[k][1] + wsum[k][2] + wsum[k][3];
   }
-  if (blk == nblk - 1 && tid == 0) p.total[f] = base;
 }
 
 // Partial products over one piece of offset f's pair list: rows a[list_a] and
