@@ -57,17 +57,21 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
   const uint32_t kvmask = p.kv >= 32 ? 0xffffffffu : ((1u << p.kv) - 1u);
   // per-task row data: row mask, pair column (argsort) and output row of the
   // lane's row (lr); rows past m read as empty
+  // Row data load as unconditional buffer loads (rows past m read 0), so a
+  // prefetch is waited on only where it is used: a load inside a branch, or
+  // a select on a loaded value, makes the compiler wait right there for
+  // every outstanding load.  The launcher takes this kernel only for masked
+  // convs, whose pair column and output row are both the argsort.
   struct Meta { uint32_t m; int col, orow; };
+  const uint32_t mbytes = (uint32_t)p.m * 4u;
+  const __amdgpu_buffer_rsrc_t rmask = make_rsrc(p.mask, mbytes);
+  const __amdgpu_buffer_rsrc_t rcol = make_rsrc(p.pair_col, mbytes);
   auto load_meta = [&](int t, Meta& M) {
     const int r = t * kMaskTile + lr;
-    M.m = 0u;
-    M.col = 0;
-    M.orow = 0;
-    if (t < ntiles && r < p.m) {
-      M.m = p.mask ? p.mask[r] : kvmask;
-      M.col = p.pair_col ? p.pair_col[r] : r;
-      M.orow = p.out_row ? p.out_row[r] : r;
-    }
+    const uint32_t off = t < ntiles && r < p.m ? (uint32_t)r * 4u : kBufOOB;
+    M.m = __builtin_amdgcn_raw_buffer_load_b32(rmask, off, 0, 0);
+    M.col = (int)__builtin_amdgcn_raw_buffer_load_b32(rcol, off, 0, 0);
+    M.orow = M.col;
   };
 
   // Block b's pool is tiles b, b + G, b + 2G, ... (G = gridDim.x: a uniform
@@ -92,13 +96,8 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
   M0.col = 0;
   M0.orow = 0;
   if (ordered) {
-    const int k = tid >> 4;
-    const int64_t r = ((int64_t)blockIdx.x + (int64_t)gridDim.x * k) * kMaskTile + (tid & 15);
-    if (k < P && r < p.m) {
-      M0.m = p.mask ? p.mask[r] : kvmask;
-      M0.col = p.pair_col ? p.pair_col[r] : (int)r;
-      M0.orow = p.out_row ? p.out_row[r] : (int)r;
-    }
+    const int k = tid >> 4;  // pool entry k is tile blockIdx.x + G * k (past ntiles when k >= P)
+    load_meta(k < P ? (int)blockIdx.x + (int)gridDim.x * k : ntiles, M0);
   }
   int t0 = ntiles, t1 = ntiles;
   Meta Mc, Mn;
@@ -133,7 +132,7 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
     m |= __shfl_xor(m, 4);
     m |= __shfl_xor(m, 8);
     uint32_t km = __builtin_amdgcn_readfirstlane(m) & p.mask_filter & kvmask;
-    if (blockIdx.y == 0 && p.mask_out != nullptr && lane == 0) p.mask_out[t] = km;
+    if (blockIdx.y == 0 && p.mask_out != nullptr && lane == 0 && t < ntiles) p.mask_out[t] = km;
     return km;
   };
   auto prepare = [&](uint32_t km, bool& live) -> uint32_t {
@@ -145,6 +144,11 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
   auto slot_f = [](uint32_t bm, int i) -> int {
     for (int j = 0; j < i; ++j) bm &= bm - 1u;
     return bm ? __builtin_ctz(bm) : -1;
+  };
+  // byte offset of pair[f][col]; f < 0 sets bit 31 (past the extent: reads 0)
+  const __amdgpu_buffer_rsrc_t rpair = make_rsrc(p.pairs, (uint32_t)((int64_t)p.kv * p.pair_ld * 4));
+  auto pair_off = [&](int f, int col) -> uint32_t {
+    return (uint32_t)(((int64_t)f * p.pair_ld + col) * 4) | ((uint32_t)(f >> 31) << 31);
   };
   // next batch: the first BATCH offsets of rem (returned as a bit mask);
   // pair indices issued (lane group lg loads slot lg, and lg + 4 when BATCH is
@@ -165,14 +169,10 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
     else fa = lg == 0 ? fs[0] : lg == 1 ? fs[1 % BATCH] : -1;
     int fb = -1;
     if constexpr (BATCH == 8) fb = lg == 0 ? fs[4] : lg == 1 ? fs[5] : lg == 2 ? fs[6] : fs[7];
-    if (p.pairs) {
-      ia = p.pairs[(int64_t)(fa < 0 ? 0 : fa) * p.pair_ld + col];
-      ib = -1;
-      if constexpr (BATCH == 8) ib = p.pairs[(int64_t)(fb < 0 ? 0 : fb) * p.pair_ld + col];
-    } else {
-      ia = col;
-      ib = col;
-    }
+    // unconditional buffer loads (slot past the batch: offset past the extent)
+    ia = (int)__builtin_amdgcn_raw_buffer_load_b32(rpair, pair_off(fa, col), 0, 0);
+    ib = -1;
+    if constexpr (BATCH == 8) ib = (int)__builtin_amdgcn_raw_buffer_load_b32(rpair, pair_off(fb, col), 0, 0);
     return bm;
   };
   // gathered rows of a batch: slot i's index for row lr sits in lane (i % 4) * 16 + lr
@@ -185,7 +185,10 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
 #pragma unroll
       for (int kc = 0; kc < KC; ++kc) {
         const int k = kc * CH + lg * E;
-        const uint32_t off = idx >= 0 ? (uint32_t)(((int64_t)idx * p.lda + k) * (int64_t)sizeof(T)) : kBufOOB;
+        // idx < 0 sets bit 31: an offset past the extent, which reads zero.
+        // Arithmetic rather than a select keeps the load out of a branch.
+        const uint32_t off = (uint32_t)(((int64_t)idx * p.lda + k) * (int64_t)sizeof(T)) |
+                             ((uint32_t)(idx >> 31) << 31);
         af[i][kc] = bld16(ra, off);
       }
     }
@@ -309,9 +312,9 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
         const int k = k0 + (tid >> 4);
         uint32_t m = M0.m;
         if (k0 > 0) {
-          m = 0u;
-          const int64_t r = ((int64_t)blockIdx.x + (int64_t)gridDim.x * k) * kMaskTile + (tid & 15);
-          if (k < P && r < p.m) m = p.mask ? p.mask[r] : kvmask;
+          Meta Mk;
+          load_meta(k < P ? (int)blockIdx.x + (int)gridDim.x * k : ntiles, Mk);
+          m = Mk.m;
         }
         m |= __shfl_xor(m, 1);
         m |= __shfl_xor(m, 2);
@@ -348,10 +351,8 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
     first_meta(wave, t0, Mc);
     first_meta(kWsWaves + wave, t1, Mn);
     __syncthreads();  // the stage buffers are free again
-    if (t0 < ntiles) {
-      rem_c = prepare(tile_mask(t0, Mc.m), live_c);
-      bm_c = batch(rem_c, Mc.col, ia_c, ib_c);
-    }
+    rem_c = prepare(tile_mask(t0, Mc.m), live_c);  // empty past the last tile
+    bm_c = batch(rem_c, Mc.col, ia_c, ib_c);
   };
   if (p.b_kmajor) {
     // B element (f, n, k) at b[k * b_ks + f * b_fs + n] (dgrad reads the KRSC
@@ -360,7 +361,7 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
     constexpr int S = (int)sizeof(T);
     constexpr int NU = NCOL / E, KU = C / E;
     constexpr int UPT = (32 * NU * KU + 64 * kWsWaves - 1) / (64 * kWsWaves);  // kv <= 32
-    const T* B = reinterpret_cast<const T*>(p.b);
+    const __amdgpu_buffer_rsrc_t rb = make_rsrc(p.b, p.b_bytes);
     const int units = p.kv * NU * KU;
     u32x4 v[UPT][E];
 #pragma unroll
@@ -370,7 +371,8 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
       const bool ok = u < units && n0 + nu * E < p.nout;
 #pragma unroll
       for (int i = 0; i < E; ++i)
-        v[j][i] = ok ? ld16(B + (int64_t)(ku * E + i) * p.b_ks + (int64_t)f * p.b_fs + n0 + nu * E) : zero16();
+        v[j][i] = bld16(rb, ok ? (uint32_t)(((int64_t)(ku * E + i) * p.b_ks + (int64_t)f * p.b_fs + n0 + nu * E) * S)
+                             : kBufOOB);
     }
     first_tile();
 #pragma unroll
@@ -398,7 +400,7 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
       }
     }
   } else {
-    const T* B = reinterpret_cast<const T*>(p.b);
+    const __amdgpu_buffer_rsrc_t rb = make_rsrc(p.b, p.b_bytes);
     const int total = p.kv * NCOL * CHUNKS;
     constexpr int WPT = (kWsLdsBytes / 16 + 64 * kWsWaves - 1) / (64 * kWsWaves);
     u32x4 v[WPT];
@@ -409,7 +411,8 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
       const int n = (i / CHUNKS) % NCOL;
       const int f = i / (CHUNKS * NCOL);
       const bool ok = i < total && n0 + n < p.nout;
-      v[j] = ok ? ld16(B + (int64_t)f * p.b_fs + (int64_t)(n0 + n) * p.b_ns + c * E) : zero16();
+      v[j] = bld16(rb, ok ? (uint32_t)(((int64_t)f * p.b_fs + (int64_t)(n0 + n) * p.b_ns + c * E) * (int)sizeof(T))
+                          : kBufOOB);
     }
     first_tile();
 #pragma unroll
@@ -437,26 +440,25 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
     uint32_t rem_n = 0u, bm_n = 0u;
     int ia_n = -1, ib_n = -1;
     Meta Mnn;
-    if (tn < ntiles) {
-      rem_n = prepare(tile_mask(tn, Mn.m), live_n);
-      bm_n = batch(rem_n, Mn.col, ia_n, ib_n);
-      load_meta(tnn, Mnn);
-    }
+    // unconditional: past the last tile the row data reads empty and the
+    // batch is empty, so no load sits behind a branch
+    rem_n = prepare(tile_mask(tn, Mn.m), live_n);
+    bm_n = batch(rem_n, Mn.col, ia_n, ib_n);
+    load_meta(tnn, Mnn);
     // a tile with more than BATCH offsets: the pair indices of each further
     // batch load while the rows of the batch before it are gathered, so every
     // further batch costs one round trip
     uint32_t bm_x = 0u;
     int ia_x = -1, ib_x = -1;
-    if (rem_c) bm_x = batch(rem_c, Mc.col, ia_x, ib_x);
+    bm_x = batch(rem_c, Mc.col, ia_x, ib_x);
     f32x4 acc[NBC];
 #pragma unroll
     for (int nb = 0; nb < NBC; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
     mma_batch(bm_c, af, acc);
     while (bm_x) {
       gather(bm_x, ia_x, ib_x, af);
-      uint32_t bm_y = 0u;
       int ia_y = -1, ib_y = -1;
-      if (rem_c) bm_y = batch(rem_c, Mc.col, ia_y, ib_y);
+      const uint32_t bm_y = batch(rem_c, Mc.col, ia_y, ib_y);
       mma_batch(bm_x, af, acc);
       bm_x = bm_y;
       ia_x = ia_y;
@@ -530,7 +532,10 @@ static hipError_t launch_ws(const RowGatherArgs& a, int max_ncol, hipStream_t s,
   *used = false;
   int ncol = std::min(ws_ncol<T>(a.kv, a.kred), max_ncol);
   if (ncol == 48) ncol = 32;
-  if (ncol == 0 || a.pair_col == nullptr) return hipSuccess;
+  // masked convs only: row data = mask + argsort (pair column = output row)
+  if (ncol == 0 || a.pair_col == nullptr || a.mask == nullptr || a.out_row != a.pair_col || a.pairs == nullptr ||
+      (int64_t)a.kv * a.pair_ld * 4 >= ((int64_t)1 << 31))
+    return hipSuccess;
   const int kc = a.kred / Mma<T>::kChunk;
   if (kc == 4 && ncol > 32) ncol = 32;
   *used = true;
