@@ -327,13 +327,15 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
         }
       }
       __syncthreads();
-      if (tid == 0) {
-        int acc = 0;
-        for (int i = 0; i < 33; ++i) {
-          const int h = hist[i];
-          hist[i] = acc;
-          acc += h;
+      if (tid < 64) {  // exclusive prefix of the 33 buckets: one wave scan
+        const int h = tid < 33 ? hist[tid] : 0;
+        int incl = h;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+          const int u = __shfl_up(incl, d);
+          if (tid >= d) incl += u;
         }
+        if (tid < 33) hist[tid] = incl - h;
       }
       __syncthreads();
       for (int k = tid; k < P; k += 64 * kWsWaves) pool[atomicAdd(&hist[32 - pcost[k]], 1)] = (uint16_t)k;
