@@ -138,7 +138,9 @@ class DeviceLayer:
             self.din = torch.zeros((t.n_in, C), dtype=td, device=dev)
         else:
             self.dw = self.dout = self.din = None
-        self.mask_out = torch.zeros(((t.n_out + 15) // 16 + 1,), dtype=torch.int32, device=dev)
+        # the forward writes every tile's mask; zero-filled only where wgrad reads it
+        alloc = torch.zeros if train else torch.empty
+        self.mask_out = alloc(((t.n_out + 15) // 16 + 1,), dtype=torch.int32, device=dev)
         self.stream = stream
         self.x = None
         self.params = {}

@@ -70,6 +70,7 @@ class ConvMainUnitTest:
 
 
 _CATALOGUE: List[ConvAlgoDesp] = []
+_SELECTED: dict = {}  # (op, dtypes, dynamic_mask) -> catalogue entry
 
 
 def select_conv_algo(op_type: int, dtype_input: int, dtype_weight: int, dtype_output: int,
@@ -82,6 +83,10 @@ def select_conv_algo(op_type: int, dtype_input: int, dtype_weight: int, dtype_ou
     if not _CATALOGUE:
         _CATALOGUE.extend(ConvMainUnitTest.get_all_conv_algo_desp())
     op = int(op_type)
+    key = (op, int(dtype_input), int(dtype_weight), int(dtype_output), bool(dynamic_mask))
+    hit = _SELECTED.get(key)
+    if hit is not None:
+        return copy.deepcopy(hit)
     bn = 64
     best = None
     for d in _CATALOGUE:
@@ -90,9 +95,11 @@ def select_conv_algo(op_type: int, dtype_input: int, dtype_weight: int, dtype_ou
         if (d.dtype_input, d.dtype_weight, d.dtype_output) != (dtype_input, dtype_weight, dtype_output):
             continue
         if op == 2 or d.tile_shape[1] == bn:
-            return copy.deepcopy(d)
+            best = d
+            break
         best = best or d
     if best is None:
         raise ValueError(f"no kernel for op {op} with dtypes input={tv.dtype_short_str(dtype_input)} "
                          f"weight={tv.dtype_short_str(dtype_weight)} output={tv.dtype_short_str(dtype_output)}")
+    _SELECTED[key] = best
     return copy.deepcopy(best)

@@ -76,6 +76,17 @@ class GemmAlgoDesp:
         object.__setattr__(self, "_abi_conv", None)
         object.__setattr__(self, name, value)
 
+    def __deepcopy__(self, memo):
+        # fields are scalars, enums, tuples and short int lists; the cached
+        # C-ABI structs are not copied (the copy rebuilds its own)
+        c = self.__class__.__new__(self.__class__)
+        d = {k: (list(v) if isinstance(v, list) else v) for k, v in self.__dict__.items()}
+        d["_abi_gemm"] = None
+        d["_abi_conv"] = None
+        c.__dict__.update(d)
+        memo[id(self)] = c
+        return c
+
     def __init__(self):
         self.dtype_a = unknown
         self.dtype_b = unknown
