@@ -165,20 +165,22 @@ __global__ void strided_cand_keys(const int32_t* __restrict__ coords, int64_t n,
 
 __global__ void decode_keys(const uint64_t* __restrict__ keys, const int64_t* n_sel, int64_t cap, Shape3 s,
                             int32_t* coords) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= cap || i >= *n_sel) return;
-  uint64_t k = keys[i];
-  const int64_t x = (int64_t)(k % (uint64_t)s.s2);
-  k /= (uint64_t)s.s2;
-  const int64_t y = (int64_t)(k % (uint64_t)s.s1);
-  k /= (uint64_t)s.s1;
-  const int64_t z = (int64_t)(k % (uint64_t)s.s0);
-  const int64_t b = (int64_t)(k / (uint64_t)s.s0);
-  int32_t* c = coords + 4 * i;
-  c[0] = (int32_t)b;
-  c[1] = (int32_t)z;
-  c[2] = (int32_t)y;
-  c[3] = (int32_t)x;
+  const int64_t n = cap < *n_sel ? cap : *n_sel;
+  // grid-stride: the launch caps the grid at 65535 blocks
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t k = keys[i];
+    const int64_t x = (int64_t)(k % (uint64_t)s.s2);
+    k /= (uint64_t)s.s2;
+    const int64_t y = (int64_t)(k % (uint64_t)s.s1);
+    k /= (uint64_t)s.s1;
+    const int64_t z = (int64_t)(k % (uint64_t)s.s0);
+    const int64_t b = (int64_t)(k / (uint64_t)s.s0);
+    int32_t* c = coords + 4 * i;
+    c[0] = (int32_t)b;
+    c[1] = (int32_t)z;
+    c[2] = (int32_t)y;
+    c[3] = (int32_t)x;
+  }
 }
 
 // ------------------------------------------------------------ workspace carving
@@ -306,10 +308,10 @@ int strided_out_impl(const int32_t* coords, int64_t n, const int32_t* in_shape, 
   }
   *n_out = cnt;
   if (cnt > cap) return fail("output coordinate buffer too small");
-  PCHK(hipMemcpyAsync(nsel, &cnt, sizeof(int64_t), hipMemcpyHostToDevice, s));
+  // the device count is >= cnt (rule 1 drops the empty marker on the host);
+  // decode_keys stops at min(cnt, *nsel) = cnt, queued on the stream
   hipLaunchKernelGGL(decode_keys, grid1(cnt), dim3(256), 0, s, k2, nsel, cnt, os, out_coords);
   PCHK(hipGetLastError());
-  PCHK(hipStreamSynchronize(s));
   return 0;
 }
 

@@ -107,7 +107,12 @@ def strided_out_coords(coords: torch.Tensor, shape: Sequence[int], rule: str = "
     _check(L.cs_pairs_strided_out(coords.data_ptr(), n, _shape_ptr(shape), r, out.data_ptr(), cap,
                                   C.byref(n_out), ws.data_ptr(), ws.numel(), s))
     out_shape = tuple((int(v) - 1) // 2 + 1 for v in shape)
-    return out[:n_out.value].clone(), out_shape
+    oc = out[:n_out.value]  # written by a kernel queued on stream s
+    if r == 1:
+        # the 27n-row buffer is mostly unused: keep a tight copy, ordered on the same stream
+        with torch.cuda.stream(torch.cuda.ExternalStream(s)):
+            oc = oc.clone()
+    return oc, out_shape
 
 
 def strided_tables(coords: torch.Tensor, shape: Sequence[int], rule: str = "down",
