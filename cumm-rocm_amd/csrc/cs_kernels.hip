@@ -125,14 +125,21 @@ __device__ __forceinline__ int wgrad_total(const WgradArgs& p, int f) {
 __global__ __launch_bounds__(256) void wgrad_rowmask_kernel(WgradArgs p) {
   const int r = blockIdx.x * 256 + threadIdx.x;
   if (r >= p.nrows) return;
-  const int i = p.a_rows[r];
-  if ((unsigned)i >= (unsigned)p.nrows) return;
+  // the argsort entry and the group's mask words load together (unconditional
+  // buffer loads; a group past num_mask reads 0), so the store waits one trip
   const int mi = (int)((int64_t)r / p.mask_width);
-  for (int w = 0; w < p.mw; ++w) {
-    uint32_t m = p.mask_filter;
-    if (p.mask != nullptr) m &= mi < p.num_mask ? p.mask[(int64_t)mi * p.mw + w] : 0u;
-    p.rowmask[(int64_t)i * p.mw + w] = m;
-  }
+  const __amdgpu_buffer_rsrc_t rm =
+      make_rsrc(p.mask, p.mask != nullptr ? (uint32_t)p.num_mask * (uint32_t)p.mw * 4u : 0u);
+  const int i = p.a_rows[r];
+  uint32_t mv[kMaxMaskWords];
+#pragma unroll
+  for (int w = 0; w < kMaxMaskWords; ++w)
+    mv[w] = __builtin_amdgcn_raw_buffer_load_b32(
+        rm, w < p.mw && mi < p.num_mask ? ((uint32_t)mi * (uint32_t)p.mw + (uint32_t)w) * 4u : kBufOOB, 0, 0);
+  if ((unsigned)i >= (unsigned)p.nrows) return;
+#pragma unroll
+  for (int w = 0; w < kMaxMaskWords; ++w)
+    if (w < p.mw) p.rowmask[(int64_t)i * p.mw + w] = p.mask_filter & (p.mask != nullptr ? mv[w] : 0xffffffffu);
 }
 
 // Pair-list construction, two passes over [kv] x [blocks of kWgradListRows
