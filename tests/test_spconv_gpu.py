@@ -288,10 +288,14 @@ def test_wide_layers_streamed_weights(dt):
     assert _rel(din.double().cpu().numpy(), quantize(exp, dt)) <= TOL[dt]
 
 
-def test_streamed_equals_weight_stationary():
+@pytest.mark.parametrize("n", [2000, 200000])
+def test_streamed_equals_weight_stationary(n):
     """The streamed-weights kernel (block_n 128) and the weight-stationary one
-    (block_n 64) give the same bits: each row sums its offsets in order."""
-    c = subm_case(2000, 64, 64, seed=23)
+    (block_n 64) give the same bits: each row sums its offsets in order.  At
+    200 000 rows each weight-stationary block orders a pool of more than 64
+    tiles (row data of the later entries loads in the ordering pass)."""
+    print(f"[stream-vs-ws] n={n}", flush=True)
+    c = subm_case(n, 64, 64, seed=23, grid=(16, 96, 96) if n <= 20000 else (41, 1600, 1408))
     prob = c.prob
     outs = []
     for bn in (64, 128):
