@@ -203,7 +203,7 @@ class SparseConvServer:
             if not self.pump():
                 time.sleep(5e-5)
 
-    # ------------------------------------------------------------ trace driver
+    # ------------------------------------------------------------ trace replay
     def run_trace(self, frames: Sequence[Frame], arrivals: Sequence[float]) -> dict:
         """Replays ``frames`` arriving at ``arrivals`` (seconds from start);
         returns latency / throughput statistics."""
