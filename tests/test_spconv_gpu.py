@@ -334,3 +334,41 @@ def test_wgrad_wide_tiles(C, K, dt):
     exp = ref.conv_wgrad(x64, d64, prob.pair_fwd, mo, prob.argsort, 16)
     tol = 1e-4 if dt == "e4m3" else max(TOL[dt], TOL[out_dt]) * 2
     assert _rel(dw.double().cpu().numpy(), quantize(exp, out_dt)) <= tol
+
+
+@pytest.fixture(scope="module")
+def case_c2():
+    # the benchmark workload: subm 3x3x3, 120 000 voxels on 8 planes in 41 x 1600 x 1408, 64 -> 64
+    return subm_case(120000, 64, 64, seed=5, grid=pc.GRID_KITTI)
+
+
+@pytest.mark.parametrize("dt", ["f32", "f16", "bf16"])
+def test_full_size_forward_dgrad_wgrad(case_c2, dt):
+    """Forward, dgrad (reversed forward masks) and wgrad (the forward's
+    mask_output, mask_width = its tile) at 120 000 voxels against the FP64
+    oracle, same tolerances as the small cases; mask_output bit-exact."""
+    c = case_c2
+    prob = c.prob
+    print(f"[full-size] dt={dt} N={prob.n} P={prob.num_pairs}", flush=True)
+    xq, wq, dq = quantize(c.X, dt), quantize(c.W, dt), quantize(c.dO, dt)
+    out, mo, fdesp = run_fwd(c, dt)
+    tile = fdesp.tile_shape[0]
+    exp, tm = ref.conv_forward(xq, wq, prob.pair_fwd, prob.mask_sorted, prob.argsort, tile)
+    assert np.array_equal(mo, tm), "mask_output must be bit-exact"
+    assert _rel(out, quantize(exp, dt)) <= TOL[dt]
+
+    din = torch.zeros((prob.n, c.C), dtype=TORCH_DT[dt], device="cuda")
+    p = conv_params(1, desp_for(1, dt, c.C, c.K), din, to_dev(c.W, dt), to_dev(c.dO, dt), prob.pair_bwd,
+                    prob.mask_sorted, prob.argsort, reverse_mask=True)
+    ConvMainUnitTest.implicit_gemm2(p)
+    torch.cuda.synchronize()
+    exp = ref.conv_dgrad(dq, wq, prob.pair_bwd, prob.mask_sorted, prob.argsort, 16, reverse_mask=True)
+    assert _rel(din.double().cpu().numpy(), quantize(exp, dt)) <= TOL[dt]
+
+    dw = torch.zeros((c.K, prob.kv, c.C), dtype=TORCH_DT[dt], device="cuda")
+    p = conv_params(2, desp_for(2, dt, c.C, c.K), to_dev(c.X, dt), dw, to_dev(c.dO, dt), prob.pair_fwd, mo,
+                    prob.argsort, mask_width=tile)
+    ConvMainUnitTest.implicit_gemm2(p)
+    torch.cuda.synchronize()
+    exp = ref.conv_wgrad(xq, dq, prob.pair_fwd, mo, prob.argsort, tile)
+    assert _rel(dw.double().cpu().numpy(), quantize(exp, dt)) <= TOL[dt] * 2
