@@ -338,18 +338,22 @@ def test_wgrad_wide_tiles(C, K, dt):
 
 @pytest.fixture(scope="module")
 def case_c2():
-    # the benchmark workload: subm 3x3x3, 120 000 voxels on 8 planes in 41 x 1600 x 1408, 64 -> 64
+    # subm 3x3x3, 120 000 voxels on 8 planes in 41 x 1600 x 1408, 64 -> 64
     return subm_case(120000, 64, 64, seed=5, grid=pc.GRID_KITTI)
 
 
-@pytest.mark.parametrize("dt", ["f32", "f16", "bf16"])
-def test_full_size_forward_dgrad_wgrad(case_c2, dt):
+@pytest.fixture(scope="module")
+def case_c3():
+    # subm 3x3x3, 200 000 voxels on 8 planes in 41 x 1600 x 1408, 128 -> 128
+    return subm_case(200000, 128, 128, seed=6, grid=pc.GRID_KITTI)
+
+
+def _full_size(c, dt):
     """Forward, dgrad (reversed forward masks) and wgrad (the forward's
-    mask_output, mask_width = its tile) at 120 000 voxels against the FP64
-    oracle, same tolerances as the small cases; mask_output bit-exact."""
-    c = case_c2
+    mask_output, mask_width = its tile) against the FP64 oracle, same
+    tolerances as the small cases; mask_output bit-exact."""
     prob = c.prob
-    print(f"[full-size] dt={dt} N={prob.n} P={prob.num_pairs}", flush=True)
+    print(f"[full-size] dt={dt} N={prob.n} P={prob.num_pairs} C={c.C} K={c.K}", flush=True)
     xq, wq, dq = quantize(c.X, dt), quantize(c.W, dt), quantize(c.dO, dt)
     out, mo, fdesp = run_fwd(c, dt)
     tile = fdesp.tile_shape[0]
@@ -372,3 +376,17 @@ def test_full_size_forward_dgrad_wgrad(case_c2, dt):
     torch.cuda.synchronize()
     exp = ref.conv_wgrad(xq, dq, prob.pair_fwd, mo, prob.argsort, tile)
     assert _rel(dw.double().cpu().numpy(), quantize(exp, dt)) <= TOL[dt] * 2
+
+
+@pytest.mark.parametrize("dt", ["f32", "f16", "bf16"])
+def test_full_size_forward_dgrad_wgrad(case_c2, dt):
+    """The 64-channel benchmark layer at full size (weight-stationary kernel
+    for the 16-bit forms)."""
+    _full_size(case_c2, dt)
+
+
+@pytest.mark.parametrize("dt", ["f32", "f16", "bf16"])
+def test_full_size_128_channels(case_c3, dt):
+    """The 128-channel layer at full size (streamed-weights kernel for the
+    16-bit forms, block kernel for FP32)."""
+    _full_size(case_c3, dt)
