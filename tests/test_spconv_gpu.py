@@ -390,3 +390,52 @@ def test_full_size_128_channels(case_c3, dt):
     """The 128-channel layer at full size (streamed-weights kernel for the
     16-bit forms, block kernel for FP32)."""
     _full_size(case_c3, dt)
+
+
+def test_full_size_128_channels_fp8(case_c3):
+    """FP8-E4M3 at 200 000 voxels, 128 -> 128, the C3 forms: forward (weights
+    per output channel, f16 out), dgrad (weights per input channel, f16 out)
+    and wgrad (FP32 dW, alpha = s_do * s_x) against the FP64 oracle on the
+    dequantised operands, same bars as the small FP8 cases."""
+    c = case_c3
+    prob = c.prob
+    print(f"[full-size fp8] N={prob.n} P={prob.num_pairs}", flush=True)
+    s_x = np.abs(c.X).max() / 448.0
+    s_do = np.abs(c.dO).max() / 448.0
+    s_wk = np.abs(c.W).reshape(c.K, -1).max(axis=1) / 448.0
+    s_wc = np.abs(c.W).max(axis=(0, 1)) / 448.0
+    xq, xq64 = _e4m3_quant(c.X, s_x)
+    dq, dq64 = _e4m3_quant(c.dO, s_do)
+    wkq, wkq64 = _e4m3_quant(c.W, s_wk[:, None, None])
+    wcq, wcq64 = _e4m3_quant(c.W, s_wc[None, None, :])
+
+    desp = desp_for(0, "e4m3", c.C, c.K, "f16")
+    tile = desp.tile_shape[0]
+    out = torch.zeros((prob.n, c.K), dtype=torch.float16, device="cuda")
+    mo = torch.zeros(((prob.n + tile - 1) // tile,), dtype=torch.int32, device="cuda")
+    scale_f = (s_x * s_wk).astype(np.float32)
+    p = conv_params(0, desp, xq.cuda(), wkq.cuda(), out, prob.pair_fwd, prob.mask_sorted, prob.argsort,
+                    mask_output=mo, scale=torch.from_numpy(scale_f).cuda())
+    ConvMainUnitTest.implicit_gemm2(p)
+    torch.cuda.synchronize()
+    exp, tm = ref.conv_forward(xq64, wkq64, prob.pair_fwd, prob.mask_sorted, prob.argsort, tile, scale=scale_f)
+    mo = mo.cpu().numpy().view(np.uint32)
+    assert np.array_equal(mo, tm), "mask_output must be bit-exact"
+    assert _rel(out.double().cpu().numpy(), quantize(exp, "f16")) <= TOL["f16"]
+
+    din = torch.zeros((prob.n, c.C), dtype=torch.float16, device="cuda")
+    scale_d = (s_do * s_wc).astype(np.float32)
+    p = conv_params(1, desp_for(1, "e4m3", c.C, c.K, "f16"), din, wcq.cuda(), dq.cuda(), prob.pair_bwd,
+                    prob.mask_sorted, prob.argsort, reverse_mask=True, scale=torch.from_numpy(scale_d).cuda())
+    ConvMainUnitTest.implicit_gemm2(p)
+    torch.cuda.synchronize()
+    exp = ref.conv_dgrad(dq64, wcq64, prob.pair_bwd, prob.mask_sorted, prob.argsort, 16, reverse_mask=True)
+    assert _rel(din.double().cpu().numpy(), quantize(exp * scale_d[None, :], "f16")) <= TOL["f16"]
+
+    dw = torch.zeros((c.K, prob.kv, c.C), dtype=torch.float32, device="cuda")
+    p = conv_params(2, desp_for(2, "e4m3", c.C, c.K, "f32"), xq.cuda(), dw, dq.cuda(), prob.pair_fwd, mo,
+                    prob.argsort, mask_width=tile, alpha=float(s_do * s_x))
+    ConvMainUnitTest.implicit_gemm2(p)
+    torch.cuda.synchronize()
+    exp = ref.conv_wgrad(xq64, dq64, prob.pair_fwd, mo, prob.argsort, tile) * (s_do * s_x)
+    assert _rel(dw.double().cpu().numpy(), exp) <= 1e-4
