@@ -498,12 +498,11 @@ namespace cs {
 
 template <typename T, typename OutT, int KC>
 static hipError_t launch_ws_kc(const RowGatherArgs& a, int ncol, hipStream_t s) {
-  static int ncu = 0;
-  if (ncu == 0) {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
-      ncu = 256;
-  }
+  // CUs of the current device, queried per launch (the runtime caches the
+  // attribute), so calls on different devices or threads share no state
+  int ncu = 0, dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
+    ncu = 256;
   const int ntiles = (a.m + kMaskTile - 1) / kMaskTile;
   // one block per CU in total: the column slices share the CUs
   const int slices = (a.nout + ncol - 1) / ncol;
