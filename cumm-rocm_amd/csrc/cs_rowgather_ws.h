@@ -12,7 +12,7 @@
 //     chunks XOR-swizzled by row so MFMA operand reads are conflict-free);
 //   * each wave takes 16-row mask tiles round-robin over the whole grid;
 //   * per tile: one load round trip for row masks / argsort, one for the pair
-//     indices of a batch of 8 / 3 / 2 offsets (1 / 2 / 4 reduction chunks;
+//     indices of a batch of 3 / 3 / 2 offsets (1 / 2 / 4 reduction chunks;
 //     each lane group loads a different offset), one for the gathered rows of
 //     those offsets (raw buffer
 //     loads, index -1 reads zero); MFMAs take weights from LDS.
@@ -38,11 +38,11 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
   constexpr int C = KC * CH;            // reduction length handled (== p.kred)
   constexpr int CHUNKS = C / E;         // 16-byte chunks per weight row
   static_assert((CHUNKS & (CHUNKS - 1)) == 0, "chunk swizzle needs a power-of-two row");
-  // offsets per gather round trip (measured on MI355X: 16 waves with these
-  // batches beat 12 waves with 8-offset batches; at two reduction chunks, 3
-  // offsets beat 2, 4 and 6, where fewer registers leave the next tile's
-  // loads more room to overlap)
-  constexpr int BATCH = KC == 1 ? 8 : (KC == 2 ? 3 : 2);
+  // offsets per gather round trip (measured on MI355X: 16 waves beat 12 waves
+  // with 8-offset batches; at one and two reduction chunks 3 offsets beat 2,
+  // 4, 6 and 8, where fewer registers leave the next tile's loads more room
+  // to overlap)
+  constexpr int BATCH = KC <= 2 ? 3 : 2;
   __shared__ __attribute__((aligned(16))) char wlds[kWsLdsBytes];
   __shared__ __attribute__((aligned(16))) char stage_lds[kWsWaves * kWsStageBytes];
   __shared__ int tile_ctr;  // next entry of this block's tile pool
