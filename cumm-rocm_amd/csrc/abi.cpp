@@ -626,7 +626,9 @@ ConvPlan plan_conv(const cs_conv_params& p) {
   const int dt_written = desp.op_type == 0 ? dt_out : desp.op_type == 1 ? dt_in : dt_w;
   if (!empty(p.bias)) {
     check_device(p.bias, "bias");
-    CS_ARG(numel(p.bias) >= (desp.op_type == 0 ? K : C), "bias too short");
+    // one value per GEMM column: K (forward), C (dgrad), kv * C (wgrad, the
+    // [K, kv * C] view of the KRSC gradient; cumm/conv/kernel.py:457-460)
+    CS_ARG(numel(p.bias) >= (desp.op_type == 0 ? K : desp.op_type == 1 ? C : kv * C), "bias too short");
     if (desp.is_int8_inference)  // bias is in the compute type (float32)
       CS_ARG(p.bias.dtype == desp.gemm.dcomp, "int8 inference bias must be float32");
     else

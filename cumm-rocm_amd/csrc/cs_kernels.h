@@ -82,7 +82,7 @@ struct WgradArgs {
 
 // out[mm*o_ms + f*o_fs + nn] = act(alpha * tree_sum_b partial[f][b][mm][nn] + beta * src)
 // over the nb_f pieces of offset f (from total / fixed_total, as in the wgrad kernel);
-// src = src_is_bias ? src[nn] : src[mm*s_ms + f*s_fs + nn]  (src may alias out)
+// src = src_is_bias ? src[f*N + nn] : src[mm*s_ms + f*s_fs + nn]  (src may alias out)
 struct ReduceArgs {
   const float* partial; const int32_t* total; int32_t fixed_total, max_blocks, min_rows, kv, M, N;
   void* out; int64_t o_ms, o_fs;

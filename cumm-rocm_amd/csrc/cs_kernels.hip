@@ -467,7 +467,10 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(ReduceArgs p) {
   const int64_t oi = (int64_t)mm * p.o_ms + (int64_t)fo * p.o_fs + nn;
   if (p.beta != 0.f) {
     const OutT* sp = reinterpret_cast<const OutT*>(p.src);
-    r += p.beta * to_f32(p.src_is_bias ? sp[nn] : sp[(int64_t)mm * p.s_ms + (int64_t)fo * p.s_fs + nn]);
+    // a bias broadcasts over the GEMM rows and is indexed by the GEMM column,
+    // which for the [K, kv * C] view of the KRSC weight gradient is f * C + c
+    r += p.beta * to_f32(p.src_is_bias ? sp[(int64_t)fo * p.N + nn]
+                                       : sp[(int64_t)mm * p.s_ms + (int64_t)fo * p.s_fs + nn]);
   }
   r = apply_act(r, p.act_type, p.act_alpha);
   reinterpret_cast<OutT*>(p.out)[oi] = from_f32<OutT>(r);

@@ -10,6 +10,7 @@ coordinates of strided layers come out sorted by (b, z, y, x), as
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import dataclasses
 from typing import Optional, Sequence, Tuple
@@ -67,6 +68,18 @@ def _ws(nbytes: int, dev) -> torch.Tensor:
     return torch.empty((max(int(nbytes), 1),), dtype=torch.uint8, device=dev)
 
 
+def _on(stream: Optional[int]):
+    """(stream handle, context): every buffer the kernels write, and their
+    scratch, is allocated inside the context, i.e. on the stream the kernels
+    are queued on.  The caching allocator then reuses a freed scratch block
+    only in that stream's order, so freeing it on return is safe while the
+    kernels still run; outputs used on another stream need the caller's sync."""
+    cur = torch.cuda.current_stream()
+    if stream is None or stream == cur.cuda_stream:
+        return cur.cuda_stream, contextlib.nullcontext()
+    return stream, torch.cuda.stream(torch.cuda.ExternalStream(stream))
+
+
 def subm_tables(coords: torch.Tensor, shape: Sequence[int], stream: Optional[int] = None,
                 backward: bool = True) -> DeviceTables:
     """``backward=False`` builds forward-only tables (no pair_bwd), all that
@@ -76,16 +89,17 @@ def subm_tables(coords: torch.Tensor, shape: Sequence[int], stream: Optional[int
     assert coords.dtype == torch.int32 and coords.dim() == 2 and coords.shape[1] == 4 and coords.is_cuda
     n = coords.shape[0]
     dev = coords.device
-    s = torch.cuda.current_stream().cuda_stream if stream is None else stream
-    pf = torch.empty((27, n), dtype=torch.int32, device=dev)
-    pb = torch.empty((27, n), dtype=torch.int32, device=dev) if backward else None
-    ms = torch.empty((n,), dtype=torch.int32, device=dev)
-    am = torch.empty((n,), dtype=torch.int32, device=dev)
-    nb = C.c_int64(0)
-    _check(L.cs_pairs_subm_workspace(n, C.byref(nb)))
-    ws = _ws(nb.value, dev)
-    _check(L.cs_pairs_subm(coords.data_ptr(), n, _shape_ptr(shape), pf.data_ptr(), _ptr(pb), ms.data_ptr(),
-                           am.data_ptr(), ws.data_ptr(), ws.numel(), s))
+    s, ctx = _on(stream)
+    with ctx:
+        pf = torch.empty((27, n), dtype=torch.int32, device=dev)
+        pb = torch.empty((27, n), dtype=torch.int32, device=dev) if backward else None
+        ms = torch.empty((n,), dtype=torch.int32, device=dev)
+        am = torch.empty((n,), dtype=torch.int32, device=dev)
+        nb = C.c_int64(0)
+        _check(L.cs_pairs_subm_workspace(n, C.byref(nb)))
+        ws = _ws(nb.value, dev)
+        _check(L.cs_pairs_subm(coords.data_ptr(), n, _shape_ptr(shape), pf.data_ptr(), _ptr(pb), ms.data_ptr(),
+                               am.data_ptr(), ws.data_ptr(), ws.numel(), s))
     shp = tuple(int(v) for v in shape)
     return DeviceTables("subm", coords, coords, shp, shp, pf, pb, ms, am, None, None)
 
@@ -96,22 +110,22 @@ def strided_out_coords(coords: torch.Tensor, shape: Sequence[int], rule: str = "
     coords = coords.contiguous()
     n = coords.shape[0]
     dev = coords.device
-    s = torch.cuda.current_stream().cuda_stream if stream is None else stream
+    s, ctx = _on(stream)
     r = RULES[rule]
     cap = n if r == 0 else 27 * n
-    out = torch.empty((max(cap, 1), 4), dtype=torch.int32, device=dev)
-    nb = C.c_int64(0)
-    _check(L.cs_pairs_strided_out_workspace(n, r, C.byref(nb)))
-    ws = _ws(nb.value, dev)
-    n_out = C.c_int64(0)
-    _check(L.cs_pairs_strided_out(coords.data_ptr(), n, _shape_ptr(shape), r, out.data_ptr(), cap,
-                                  C.byref(n_out), ws.data_ptr(), ws.numel(), s))
-    out_shape = tuple((int(v) - 1) // 2 + 1 for v in shape)
-    oc = out[:n_out.value]  # written by a kernel queued on stream s
-    if r == 1:
-        # the 27n-row buffer is mostly unused: keep a tight copy, ordered on the same stream
-        with torch.cuda.stream(torch.cuda.ExternalStream(s)):
+    with ctx:
+        out = torch.empty((max(cap, 1), 4), dtype=torch.int32, device=dev)
+        nb = C.c_int64(0)
+        _check(L.cs_pairs_strided_out_workspace(n, r, C.byref(nb)))
+        ws = _ws(nb.value, dev)
+        n_out = C.c_int64(0)
+        _check(L.cs_pairs_strided_out(coords.data_ptr(), n, _shape_ptr(shape), r, out.data_ptr(), cap,
+                                      C.byref(n_out), ws.data_ptr(), ws.numel(), s))
+        oc = out[:n_out.value]  # written by a kernel queued on stream s
+        if r == 1:
+            # the 27n-row buffer is mostly unused: keep a tight copy, ordered on the same stream
             oc = oc.clone()
+    out_shape = tuple((int(v) - 1) // 2 + 1 for v in shape)
     return oc, out_shape
 
 
@@ -121,21 +135,22 @@ def strided_tables(coords: torch.Tensor, shape: Sequence[int], rule: str = "down
     L = _abi.lib()
     coords = coords.contiguous()
     dev = coords.device
-    s = torch.cuda.current_stream().cuda_stream if stream is None else stream
+    s, ctx = _on(stream)
     oc, out_shape = strided_out_coords(coords, shape, rule, s)
     n_in, n_out = coords.shape[0], oc.shape[0]
-    pf = torch.empty((27, n_out), dtype=torch.int32, device=dev)
-    pb = torch.empty((27, n_in), dtype=torch.int32, device=dev) if backward else None
-    mf = torch.empty((n_out,), dtype=torch.int32, device=dev)
-    af = torch.empty((n_out,), dtype=torch.int32, device=dev)
-    mb = torch.empty((n_in,), dtype=torch.int32, device=dev) if backward else None
-    ab = torch.empty((n_in,), dtype=torch.int32, device=dev) if backward else None
-    nb = C.c_int64(0)
-    _check(L.cs_pairs_strided_workspace(n_in, n_out, C.byref(nb)))
-    ws = _ws(nb.value, dev)
-    _check(L.cs_pairs_strided(coords.data_ptr(), n_in, _shape_ptr(shape), oc.data_ptr(), n_out, pf.data_ptr(),
-                              _ptr(pb), mf.data_ptr(), af.data_ptr(), _ptr(mb), _ptr(ab),
-                              ws.data_ptr(), ws.numel(), s))
+    with ctx:
+        pf = torch.empty((27, n_out), dtype=torch.int32, device=dev)
+        pb = torch.empty((27, n_in), dtype=torch.int32, device=dev) if backward else None
+        mf = torch.empty((n_out,), dtype=torch.int32, device=dev)
+        af = torch.empty((n_out,), dtype=torch.int32, device=dev)
+        mb = torch.empty((n_in,), dtype=torch.int32, device=dev) if backward else None
+        ab = torch.empty((n_in,), dtype=torch.int32, device=dev) if backward else None
+        nb = C.c_int64(0)
+        _check(L.cs_pairs_strided_workspace(n_in, n_out, C.byref(nb)))
+        ws = _ws(nb.value, dev)
+        _check(L.cs_pairs_strided(coords.data_ptr(), n_in, _shape_ptr(shape), oc.data_ptr(), n_out, pf.data_ptr(),
+                                  _ptr(pb), mf.data_ptr(), af.data_ptr(), _ptr(mb), _ptr(ab),
+                                  ws.data_ptr(), ws.numel(), s))
     return DeviceTables("strided", coords, oc, tuple(int(v) for v in shape), out_shape, pf, pb, mf, af, mb, ab)
 
 

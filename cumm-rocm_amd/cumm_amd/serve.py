@@ -72,6 +72,10 @@ class SparseConvServer:
         self.reserved = 0
         self.max_inflight, self.max_batch = max_inflight, max_batch
         self.streams = [torch.cuda.Stream() for _ in range(n_streams)]
+        # the weights were converted on the current stream; the pool streams
+        # (created non-blocking) must not read them before that finishes
+        for st in self.streams:
+            st.wait_stream(torch.cuda.current_stream())
         # pinned host staging per stream, grown on demand and reused (a fresh
         # pinned allocation per batch costs milliseconds)
         self._pin_c: List[Optional[torch.Tensor]] = [None] * n_streams

@@ -104,9 +104,14 @@ class ConvTuner:
         if key in self.cache:
             self.apply(p, self.cache[key])
             return self.cache[key]
-        stream = stream or torch.cuda.current_stream()
+        # default: the stream the params already run on; clones, timing and
+        # restores are ordered on it, and p.stream is left as it was
+        if stream is None:
+            stream = torch.cuda.ExternalStream(p.stream) if p.stream else torch.cuda.current_stream()
+        old_stream = p.stream
         p.stream = stream.cuda_stream
-        saved = [t.clone() for t in self._written(p)]
+        with torch.cuda.stream(stream):
+            saved = [t.clone() for t in self._written(p)]
         op = int(p.conv_algo_desp.op_type)
         splits = WGRAD_SPLITS if op == ConvOpType.BackwardWeight else (1,)
         best = None
@@ -120,9 +125,11 @@ class ConvTuner:
                 tried.append((repr(d), sk, us))
                 if best is None or us < best[2]:
                     best = (d, sk, us)
-        for t, s in zip(self._written(p), saved):
-            t.copy_(s)
+        with torch.cuda.stream(stream):
+            for t, s in zip(self._written(p), saved):
+                t.copy_(s)
         torch.cuda.synchronize()
+        p.stream = old_stream
         res = TuneResult(best[0], best[1], best[2], tried)
         self.cache[key] = res
         self.apply(p, res)

@@ -211,7 +211,7 @@ def conv_dgrad(dO, W, pair_bwd, mask_sorted, argsort, tile_m: int, mask_filter: 
 
 def conv_wgrad(X, dO, pair, mask_reduced, argsort, mask_width: int, mask_filter: int = ALL,
                alpha: float = 1.0, beta: float = 0.0, dW_init=None, kv: Optional[int] = None,
-               act_type: int = 0, act_alpha: float = 0.0) -> np.ndarray:
+               act_type: int = 0, act_alpha: float = 0.0, bias=None) -> np.ndarray:
     """Wgrad (op 2): returns dW FP64 [K, kv, C].
 
     Offsets outside mask_filter are left untouched (kernel.py:891-897).  Sorted
@@ -219,7 +219,9 @@ def conv_wgrad(X, dO, pair, mask_reduced, argsort, mask_width: int, mask_filter:
     ``mask_reduced[r // mask_width]`` (one word, or words [W] with dynamic masks) (the forward mask_output; the k-tile skip
     of mma.py:607-648 with mask_width % TK == 0) and ``pair[f][argsort[r]] != -1``:
     ``dW[:, f, :] = epi(sum_r dO[argsort[r]]^T X[pair[f][argsort[r]]])``, the
-    split-K partitions summing into one result (kernel.py:960-1083).
+    split-K partitions summing into one result (kernel.py:960-1083).  A bias is
+    a row-broadcast source indexed by the GEMM column, which for the
+    ``[K, kv * C]`` view of the gradient is ``f * C + c`` (kernel.py:457-460).
     """
     X = np.asarray(X, dtype=np.float64)
     dO = np.asarray(dO, dtype=np.float64)
@@ -242,7 +244,10 @@ def conv_wgrad(X, dO, pair, mask_reduced, argsort, mask_width: int, mask_filter:
         idx = pair[f][argsort[rows]]
         ok = idx >= 0
         acc = dO[argsort[rows[ok]]].T @ X[idx[ok]]
-        src = dW[:, f, :] if beta != 0.0 else None
+        src = None
+        if beta != 0.0:
+            src = (np.broadcast_to(np.asarray(bias, np.float64)[f * C:(f + 1) * C], (K, C)) if bias is not None
+                   else dW[:, f, :])
         dW[:, f, :] = epilogue(acc, alpha, beta, src, act_type, act_alpha)
     return dW
 

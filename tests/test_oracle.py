@@ -297,6 +297,28 @@ def test_wgrad_reduced_mask_skip_and_filter():
                                rtol=1e-12, atol=1e-12)
 
 
+def test_wgrad_epilogue_bias_is_a_row_constant_source():
+    """beta * bias in wgrad equals beta * (the output itself) when every row k
+    of the output holds the same values bias[f * C + c]: the bias is indexed by
+    the column of the [K, kv * C] view (a per-offset slice, not bias[c])."""
+    prob, X, W, dO = _tiny(seed=14)
+    K, kv, C = W.shape
+    tm = ref.tile_masks(prob.mask_sorted, prob.n, 16).astype(np.uint32)
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal(kv * C)
+    init = np.broadcast_to(b.reshape(1, kv, C), (K, kv, C)).copy()
+    via_bias = ref.conv_wgrad(X, dO, prob.pair_fwd, tm, prob.argsort, 16, alpha=0.5, beta=2.0,
+                              dW_init=np.zeros_like(W), bias=b, act_type=3, act_alpha=0.1)
+    via_out = ref.conv_wgrad(X, dO, prob.pair_fwd, tm, prob.argsort, 16, alpha=0.5, beta=2.0,
+                             dW_init=init, act_type=3, act_alpha=0.1)
+    np.testing.assert_allclose(via_bias, via_out, rtol=1e-13, atol=1e-13)
+    # a bias read as bias[c] would repeat the first offset's slice: it must not
+    wrong = ref.conv_wgrad(X, dO, prob.pair_fwd, tm, prob.argsort, 16, alpha=0.5, beta=2.0,
+                           dW_init=np.broadcast_to(b[:C].reshape(1, 1, C), (K, kv, C)).copy(),
+                           act_type=3, act_alpha=0.1)
+    assert not np.allclose(via_bias, wrong)
+
+
 # --------------------------------------------------------------- gemm
 def test_gemm_closed_forms():
     rng = np.random.default_rng(12315)

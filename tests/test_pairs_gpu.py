@@ -104,3 +104,38 @@ def test_forward_only_tables(kind):
     assert fwd.pair_bwd is None and fwd.mask_bwd is None and fwd.argsort_bwd is None
     with pytest.raises(ValueError, match="backward"):
         B.DeviceLayer(fwd, 16, 16, "f16", np.zeros((16, 27, 16), np.float32), 0)
+
+
+@pytest.mark.parametrize("kind", ["subm", "strided"])
+def test_tables_on_a_side_stream(kind):
+    """Tables built on a stream other than the current one: scratch and outputs
+    belong to that stream, so allocations made on the current stream right
+    after the call (while the kernels may still run) cannot take their memory."""
+    print(f"[pairs] side stream {kind}", flush=True)
+    grid = (16, 96, 96)
+    c = _coords([6000, 3000], grid, 9)
+    dc = torch.from_numpy(c).cuda()
+    torch.cuda.synchronize()
+    side = torch.cuda.Stream()
+    if kind == "subm":
+        dev = pairs.subm_tables(dc, grid, stream=side.cuda_stream)
+    else:
+        dev = pairs.strided_tables(dc, grid, rule="down", stream=side.cuda_stream)
+    # churn the current stream's pool while the side stream may still be busy
+    junk = [torch.full((1 << 16,), -7, dtype=torch.int32, device="cuda") for _ in range(32)]
+    side.synchronize()
+    torch.cuda.synchronize()
+    del junk
+    if kind == "subm":
+        host = pc.build_subm(c, grid)
+        assert np.array_equal(_np(dev.pair_fwd), host.pair_fwd)
+        assert np.array_equal(_np(dev.pair_bwd), host.pair_bwd)
+        assert np.array_equal(_np(dev.mask_fwd).view(np.uint32), host.mask_sorted)
+        assert np.array_equal(_np(dev.argsort_fwd), host.argsort)
+    else:
+        host = pc.build_strided(c, grid, rule="down")
+        assert np.array_equal(_np(dev.out_coords), host.out_coords)
+        assert np.array_equal(_np(dev.pair_fwd), host.pair_fwd)
+        assert np.array_equal(_np(dev.pair_bwd), host.pair_bwd)
+        assert np.array_equal(_np(dev.mask_fwd).view(np.uint32), host.mask_fwd_sorted)
+        assert np.array_equal(_np(dev.argsort_bwd), host.argsort_bwd)
