@@ -37,7 +37,10 @@ def _plane_points(rng: np.random.Generator, grid: Sequence[int], m: int,
 
     The plane passes through a uniform point of the grid box with a normal drawn
     uniformly on the sphere; points are uniform over the part of the plane that
-    lies inside the box (rejection from a square that covers the box).
+    lies inside the box: rejection from the rectangle (in plane coordinates)
+    that bounds the plane's intersection with the box, widened by 6 noise
+    standard deviations so that jittered points from just outside the box keep
+    their (negligible) chance of landing inside.
     """
     g = np.asarray(grid, dtype=np.float64)
     normal = rng.normal(size=3)
@@ -47,12 +50,30 @@ def _plane_points(rng: np.random.Generator, grid: Sequence[int], m: int,
     u /= np.linalg.norm(u)
     v = np.cross(normal, u)
     center = rng.uniform(0.0, 1.0, size=3) * g
-    half = float(np.linalg.norm(g))  # the square of side 2*|g| covers the box
+    # plane / box-edge intersections, in plane coordinates (s, t)
+    corners = np.array([[i, j, k] for i in (0, 1) for j in (0, 1) for k in (0, 1)], dtype=np.float64) * g
+    st_pts = []
+    for a in range(8):
+        for b in range(a + 1, 8):
+            d = corners[b] - corners[a]
+            if np.count_nonzero(d) != 1:
+                continue  # not an edge
+            den = float(normal @ d)
+            if abs(den) < 1e-12:
+                continue
+            lam = float(normal @ (center - corners[a])) / den
+            if 0.0 <= lam <= 1.0:
+                q = corners[a] + lam * d - center
+                st_pts.append((float(q @ u), float(q @ v)))
+    st_pts = np.asarray(st_pts)
+    pad = 6.0 * noise + 1e-6
+    lo = st_pts.min(axis=0) - pad
+    hi = st_pts.max(axis=0) + pad
     out = []
     have = 0
     while have < m:
         batch = max(4 * (m - have), 65536)
-        st = rng.uniform(-half, half, size=(batch, 2))
+        st = rng.uniform(lo, hi, size=(batch, 2))
         p = center + st[:, :1] * u + st[:, 1:] * v
         p = p + rng.normal(0.0, noise, size=p.shape)
         ok = np.all((p >= 0.0) & (p < g), axis=1)
