@@ -68,7 +68,8 @@ def test_int8_inference_bit_exact(C, K, bn):
     got = out.cpu().numpy()
     assert np.array_equal(mo.cpu().numpy().view(np.uint32)[:tiles], tm)
     assert np.array_equal(got, exp), f"{np.count_nonzero(got != exp)} of {got.size} differ"
-    assert (got == 127).any() and (np.abs(got.astype(np.int32)) < 100).any()
+    # the largest-magnitude outputs saturate (at 127 or -128, whichever sign they have)
+    assert ((got == 127) | (got == -128)).any() and (np.abs(got.astype(np.int32)) < 100).any()
 
 
 @pytest.mark.parametrize("bn", [64, 128])
