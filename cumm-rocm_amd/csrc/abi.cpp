@@ -1107,6 +1107,48 @@ int copy_params(const std::vector<GsParam>& v, int32_t* out, int cap) {
 
 }  // namespace
 
+// ====================================================================== allocation-free conv
+// Used by cs_implicit_gemm2_noalloc (cs_noalloc.hip), which owns the memory.
+namespace cs_detail {
+
+__attribute__((visibility("hidden"))) int fail_runtime(const char* msg) {
+  g_last_error = msg;
+  return CS_ERR_RUNTIME;
+}
+
+__attribute__((visibility("hidden"))) int implicit_gemm2_in(const cs_conv_params* params, void* mem, int64_t cap,
+                                                            int device) {
+  return guarded([&] {
+    CS_ARG(params != nullptr, "params is null");
+    cs_conv_params q = *params;
+    ConvPlan plan = plan_conv(q);
+    if (plan.op == 2 && plan.workspace > cap) {
+      // fewer split pieces per offset: the partition only changes for an
+      // offset with more than pieces * kWgradMinRows pairs
+      const int64_t other = wgrad_layout(plan.kv, plan.K, plan.C, plan.N, 0, true, plan.mw).size;
+      const int64_t per = plan.kv * plan.K * plan.C * 4;
+      const int64_t fit = (cap - other - 256) / std::max<int64_t>(per, 1);
+      CS_ARG(fit >= 2, "the call needs a workspace of ", plan.workspace, " bytes (at least ",
+             other + 2 * per + 256, " with split pieces capped), beyond the ", cap,
+             " bytes of the allocation-free entry point's scratch");
+      q.split_k_slices = (int)std::min<int64_t>(plan.splits, fit);
+      plan = plan_conv(q);
+    }
+    CS_ARG(plan.workspace <= cap, "the call needs a workspace of ", plan.workspace, " bytes, beyond the ", cap,
+           " bytes of the allocation-free entry point's scratch");
+    std::memset(&q.workspace, 0, sizeof(q.workspace));
+    q.workspace.data = mem;
+    q.workspace.dtype = 6;  // u8
+    q.workspace.ndim = 1;
+    q.workspace.shape[0] = cap;
+    q.workspace.stride[0] = 1;
+    q.workspace.device = device;
+    run_conv(q);
+  });
+}
+
+}  // namespace cs_detail
+
 // ====================================================================== C ABI
 extern "C" {
 

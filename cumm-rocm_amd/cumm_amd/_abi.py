@@ -23,7 +23,7 @@ ABI_VERSION = 1
 
 # exported symbols, in header order
 SYMBOLS = [
-    "cs_abi_version", "cs_implicit_gemm2", "cs_matmul2", "cs_conv_workspace_size",
+    "cs_abi_version", "cs_implicit_gemm2", "cs_implicit_gemm2_noalloc", "cs_matmul2", "cs_conv_workspace_size",
     "cs_gemm_workspace_size", "cs_get_all_conv_algo_desp", "cs_get_all_algo_desp",
     "cs_gemm_algo_name", "cs_conv_algo_name", "cs_device_synchronize",
     "cs_stream_synchronize", "cs_last_error",
@@ -109,6 +109,8 @@ def lib() -> C.CDLL:
     L.cs_abi_version.restype = C.c_int
     L.cs_implicit_gemm2.argtypes = [C.POINTER(CsConvParams)]
     L.cs_implicit_gemm2.restype = C.c_int
+    L.cs_implicit_gemm2_noalloc.argtypes = [C.POINTER(CsConvParams)]
+    L.cs_implicit_gemm2_noalloc.restype = C.c_int
     L.cs_matmul2.argtypes = [C.POINTER(CsGemmParams)]
     L.cs_matmul2.restype = C.c_int
     L.cs_conv_workspace_size.argtypes = [C.POINTER(CsConvParams), C.POINTER(C.c_int64)]

@@ -37,6 +37,15 @@ class ConvMainUnitTest:
             _abi.check(L.cs_implicit_gemm2(C.byref(cp)))
 
     @staticmethod
+    def implicit_gemm2_noalloc(params: ConvParams) -> None:
+        """``implicit_gemm2`` without any device allocation (no counterpart in
+        the reference): the workspace is the library's per-GPU scratch array
+        (``params.workspace`` is ignored).  Calls that overlap on one GPU race;
+        a call needing more than the scratch raises ValueError."""
+        cp = _abi.conv_params_to_cs(params)
+        _abi.check(_abi.lib().cs_implicit_gemm2_noalloc(C.byref(cp)))
+
+    @staticmethod
     def query_workspace_size(params: ConvParams) -> int:
         """Bytes of workspace this build's kernels need for ``params`` (a caller
         that passes an empty workspace gets one allocated per call)."""

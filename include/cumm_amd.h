@@ -115,6 +115,19 @@ int cs_abi_version(void);
  * and released after the launches. */
 int cs_implicit_gemm2(const cs_conv_params* params);
 
+/* cs_implicit_gemm2 without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The call's
+ * workspace is a file-scope device array of CS_NOALLOC_SCRATCH_BYTES held by
+ * each GPU the library runs on until the process ends; params->workspace is
+ * ignored.  A call whose workspace exceeds the array fails with
+ * CS_ERR_INVALID_ARG (wgrad first lowers its split pieces per offset to fit,
+ * which changes the FP32 summation only for an offset with more than
+ * pieces * 1024 pairs).  Every call on one GPU shares that GPU's array: calls
+ * that overlap on one GPU (other streams, other threads) race. */
+#define CS_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int cs_implicit_gemm2_noalloc(const cs_conv_params* params);
+
 /* GemmMainUnitTest.matmul2(params)  -- cumm/gemm/main.py:1176-1335 with the host
  * preamble of cumm/gemm/nvrtc_code.py:34-145.  NoShuffle, ShuffleAC (gather A /
  * scatter C) and ShuffleAB (gather A and B, sum over rows). */
