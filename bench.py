@@ -61,6 +61,8 @@ def parse():
     ap.add_argument("--dtypes", default=None)
     ap.add_argument("--ops", default="fwd,dgrad,wgrad")
     ap.add_argument("--copy-bytes", type=int, default=0)
+    ap.add_argument("--no-extra", action="store_true",
+                    help="c2 only: skip the precision, batched and serving forms in the same line")
     args = ap.parse_args()
     dflt = {"c2": (120000, 64, "f32,f16,bf16"), "c3": (200000, 128, "f32,bf16,f16,e4m3"),
             "c4": (0, 0, "f16"), "c5": (0, 0, "bf16"), "f3": (120000, 64, "i8")}[args.config]
@@ -241,17 +243,60 @@ def load_traffic(kernel: str, workload: str):
     return None
 
 
-def cpu_baseline(prob, X, W, dO, budget_s: float, gpu_outputs):
-    """The reference's CPU path (numpy FP32 gather-GEMM-scatter, compact pairs)
-    timed on the host, plus the FP64-oracle error of each GPU result."""
+def host_info() -> dict:
+    """The host the CPU baseline runs on: logical CPUs, those this process may
+    use, the CPU model and the BLAS thread count."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        from threadpoolctl import threadpool_info
+        blas = max([i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"] or [1])
+    except Exception:
+        blas = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"cpu_count": os.cpu_count(), "cpus_usable": usable, "cpu_model": model, "blas_threads": int(blas)}
+
+
+def accuracy_vs_fp64(prob, X, W, dO, gpu_outputs):
+    """Max-abs and relative error of every GPU result against the FP64 oracle
+    on the unquantised FP32 inputs, and mask_output bit-exactness."""
     sys.path.insert(0, str(ROOT))
     from oracle import spconv_ref as ref
 
-    try:
-        from threadpoolctl import threadpool_info
-        threads = max([i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"] or [1])
-    except Exception:
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    n = prob.n
+    acc = {}
+    y64, _ = ref.conv_forward(X, W, prob.pair_fwd, prob.mask_sorted, prob.argsort, 16)
+    d64 = ref.conv_dgrad(dO, W, prob.pair_bwd, prob.mask_sorted, prob.argsort, 16, reverse_mask=True)
+    tm = ref.tile_masks(prob.mask_sorted, n, 16).astype(np.uint32)
+    w64 = ref.conv_wgrad(X, dO, prob.pair_fwd, tm, prob.argsort, 16)
+    for dt_name, (y, d, w, mo) in gpu_outputs.items():
+        e = {}
+        for name, got, exp in (("fwd", y, y64), ("dgrad", d, d64), ("wgrad", w, w64)):
+            mx, rel = ref.relative_error(got, exp)
+            e[name] = {"max_abs": mx, "rel": rel}
+        e["mask_output_bit_exact"] = bool(np.array_equal(mo, tm))
+        acc[dt_name] = e
+    return acc
+
+
+def cpu_baseline(prob, X, W, dO, budget_s: float):
+    """The reference's CPU path (numpy FP32 gather-GEMM-scatter, compact pairs)
+    timed on the host."""
+    sys.path.insert(0, str(ROOT))
+    from oracle import spconv_ref as ref
+
+    host = host_info()
+    threads = host["blas_threads"]
     pairs, nums = ref.compact_pairs(prob.pair_fwd)
     n, C, K = prob.n, X.shape[1], W.shape[0]
     flops_step = 3 * 2 * prob.num_pairs * C * K
@@ -266,25 +311,13 @@ def cpu_baseline(prob, X, W, dO, budget_s: float, gpu_outputs):
         if time.perf_counter() - t0 >= budget_s:
             break
     dt = (time.perf_counter() - t0) / steps
-    # accuracy of the GPU results against the FP64 oracle (original FP32 inputs)
-    acc = {}
-    y64, _ = ref.conv_forward(X, W, prob.pair_fwd, prob.mask_sorted, prob.argsort, 16)
-    d64 = ref.conv_dgrad(dO, W, prob.pair_bwd, prob.mask_sorted, prob.argsort, 16, reverse_mask=True)
-    tm = ref.tile_masks(prob.mask_sorted, n, 16).astype(np.uint32)
-    w64 = ref.conv_wgrad(X, dO, prob.pair_fwd, tm, prob.argsort, 16)
-    for dt_name, (y, d, w, mo) in gpu_outputs.items():
-        e = {}
-        for name, got, exp in (("fwd", y, y64), ("dgrad", d, d64), ("wgrad", w, w64)):
-            mx, rel = ref.relative_error(got, exp)
-            e[name] = {"max_abs": mx, "rel": rel}
-        e["mask_output_bit_exact"] = bool(np.array_equal(mo, tm))
-        acc[dt_name] = e
     return {
         "value": flops_step / dt / 1e12, "unit": "TFLOP/s", "cores": int(threads), "kind": "port",
         "sample": f"{steps} full fwd+dgrad+wgrad steps of the same {n}-voxel {C}->{K} layer "
                   f"(numpy FP32, compact pairs), {dt * 1e3:.0f} ms/step",
         "ms_per_step": dt * 1e3,
-    }, acc
+        "host": host,
+    }
 
 
 def _clouds_worker(job):
@@ -411,7 +444,7 @@ def run_batched(args):
         "tuned": [{"op": c[0], "C": c[1], "K": c[2], "kernel": c[3], "split_k": c[4], "us": c[5]} for c in choices],
         "checks": checks,
     }
-    print(json.dumps(line))
+    return line
 
 
 def _frame_worker(job):
@@ -483,7 +516,7 @@ def run_serving(args):
                    "frame_pool_voxels": sizes, "hbm_budget_bytes": 16 << 30},
         "serving": res,
     }
-    print(json.dumps(line))
+    return line
 
 
 def run_f3(args):
@@ -594,11 +627,67 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
     if args.config == "c4":
-        return run_batched(args)
+        print(json.dumps(run_batched(args)))
+        return
     if args.config == "c5":
-        return run_serving(args)
+        print(json.dumps(run_serving(args)))
+        return
     if args.config == "f3":
         return run_f3(args)
+    line = run_single(args, with_cpu=not args.no_cpu_baseline)
+    if args.config == "c2" and not args.no_extra:
+        line["configs"] = run_extra(args)
+    print(json.dumps(line))
+
+
+def run_extra(args) -> dict:
+    """The precision (200k voxels, 128 -> 128, f32 / bf16 / f16 / e4m3),
+    batched (32 clouds under a HIP graph) and serving (Poisson arrivals, 1 s
+    per rate) forms, summarised for the same output line as the c2 headline."""
+    import copy
+
+    out = {}
+    a3 = copy.copy(args)
+    a3.config, a3.voxels, a3.channels, a3.dtypes = "c3", 200000, 128, "f32,bf16,f16,e4m3"
+    a3.steps, a3.warmup = 20, 5
+    t0 = time.perf_counter()
+    try:
+        l3 = run_single(a3, with_cpu=False)
+        out["c3"] = {"workload": l3["config"]["workload"], "pairs": l3["config"]["pairs"],
+                     "mac_efficiency": l3["config"]["mac_efficiency"],
+                     "per_dtype": {k: {"tflops": v["tflops"], "speedup_vs_f32": v["speedup_vs_f32"],
+                                       "frac_mfma_peak": v["frac_mfma_peak"], "op_us": v["op_us"]}
+                                   for k, v in l3["per_dtype"].items()},
+                     "accuracy_vs_fp64": l3["accuracy_vs_fp64"], "roofline_ops": l3["roofline_ops"]}
+    except Exception as e:  # the headline must still print
+        out["c3"] = {"error": repr(e)}
+    log(f"extra c3: {time.perf_counter() - t0:.1f}s")
+    a4 = copy.copy(args)
+    a4.steps, a4.warmup = 10, 3
+    t0 = time.perf_counter()
+    try:
+        l4 = run_batched(a4)
+        out["c4"] = {k: l4[k] for k in ("value", "unit", "ms_per_step", "config", "forms_ms", "forms_tflops",
+                                        "speedup_vs_f32_per_call", "frac_mfma_peak_f16", "checks")}
+    except Exception as e:
+        out["c4"] = {"error": repr(e)}
+    log(f"extra c4: {time.perf_counter() - t0:.1f}s")
+    a5 = copy.copy(args)
+    a5.seconds, a5.rates = 1.0, "20,50,100,200"
+    t0 = time.perf_counter()
+    try:
+        l5 = run_serving(a5)
+        out["c5"] = {"config": l5["config"], "serving": l5["serving"]}
+    except Exception as e:
+        out["c5"] = {"error": repr(e)}
+    log(f"extra c5: {time.perf_counter() - t0:.1f}s")
+    return out
+
+
+def run_single(args, with_cpu: bool) -> dict:
+    """One subm 3x3x3 layer (c2 / c3): per-dtype step time, per-op kernel
+    time, roofline of every op, FP64 error of every result, and (c2) the CPU
+    baseline."""
     import torch
     from cumm_amd import pointcloud as pc
 
@@ -673,10 +762,10 @@ def main():
                            "frac_hbm": b / (v * 1e-3) / HBM_PEAK,
                            "frac_mfma": flops_op / (v * 1e-3) / PEAK[main_dt]}
 
-    cpu = None
-    accuracy = None
-    if not args.no_cpu_baseline:
-        cpu, accuracy = cpu_baseline(prob, X, W, dO, args.cpu_seconds, gpu_outputs)
+    cpu = cpu_baseline(prob, X, W, dO, args.cpu_seconds) if with_cpu else None
+    accuracy = accuracy_vs_fp64(prob, X, W, dO, gpu_outputs)
+    # MAC efficiency of the mask tiles: valid pairs / (16 rows x offsets of each tile's mask)
+    mac_eff = pc.mac_efficiency(prob.mask_sorted, 16, P)
 
     line = {
         "metric": "sparse-conv TFLOP/s + % MFMA peak per dtype vs plain FP32; serving p50/p99 ms",
@@ -687,14 +776,15 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": main_dt, "data": "synthetic",
         "config": {"workload": workload, "ops": list(ops), "voxels": prob.n, "pairs": P, "channels_in": C,
-                   "channels_out": K, "kernel_volume": prob.kv, "flops_per_step": flops_step},
+                   "channels_out": K, "kernel_volume": prob.kv, "flops_per_step": flops_step,
+                   "mac_efficiency": mac_eff, "dense_equivalent_flops_per_op": 2 * prob.n * prob.kv * C * K},
         "per_dtype": per_dtype,
         "roofline": roofline,
         "roofline_ops": roofline_ops,
         "cpu_baseline": cpu,
         "accuracy_vs_fp64": accuracy,
     }
-    print(json.dumps(line))
+    return line
 
 
 if __name__ == "__main__":
