@@ -51,8 +51,9 @@ def parse():
                          "c4: 32 clouds, strided 32->64 + subm 64->128, grouped launches under a HIP graph; "
                          "c5: serving, Poisson 20-200 frames/s, 4-layer BF16 stack; "
                          "f3: int8-inference forward and the gather / scatter-add glue on the c2 scene")
-    ap.add_argument("--rates", default="20,50,100,200", help="c5 arrival rates (frames/s)")
+    ap.add_argument("--rates", default="20,50,100,200,400", help="c5 arrival rates (frames/s)")
     ap.add_argument("--seconds", type=float, default=3.0, help="c5 traffic per rate")
+    ap.add_argument("--forms", default="", help="c5: comma-separated serving forms to run (default: all)")
     ap.add_argument("--voxels", type=int, default=None)
     ap.add_argument("--channels", type=int, default=None)
     ap.add_argument("--seed", type=int, default=0)
@@ -458,8 +459,9 @@ def run_serving(args):
     """Serving form: Poisson arrivals at each rate, frames of 20k-200k voxels
     (a pool of 16 scenes generated as for the single-layer workload), 4-layer
     stack subm 16->32, strided 32->64, subm 64->64, strided 64->128.  The BF16
-    server batches continuously on two streams under an HBM budget; the plain
-    FP32 per-call path serves the same trace one frame at a time."""
+    server batches continuously on two streams (a launch thread each) under an
+    HBM budget, and on one stream for comparison; the plain FP32 per-call path
+    serves the same trace one frame at a time."""
     from concurrent.futures import ProcessPoolExecutor
 
     from cumm_amd import batch as B
@@ -486,14 +488,20 @@ def run_serving(args):
         arr = np.cumsum(r.exponential(1.0 / rate, n))
         return [Frame(i, pool[j], feats[j]) for i, j in enumerate(idx)], arr
 
-    forms = {"bf16_continuous": dict(dt="bf16", max_batch=16, n_streams=1),
-             "bf16_continuous_2streams": dict(dt="bf16", max_batch=16, n_streams=2),
+    forms = {"bf16_continuous": dict(dt="bf16", max_batch=16, n_streams=2),
+             "bf16_continuous_1stream": dict(dt="bf16", max_batch=16, n_streams=1),
              "f32_per_call": dict(dt="f32", max_batch=1, n_streams=1)}
+    if getattr(args, "forms", ""):
+        forms = {k: forms[k] for k in args.forms.split(",")}
     res = {}
     for name, kw in forms.items():
         srv = SparseConvServer(kinds, chans, Ws, keep_outputs=False, hbm_budget=16 << 30, **kw)
-        wf, wa = trace(1000.0, 1)
-        srv.run_trace(wf[:6], wa[:6] - wa[0])  # warm-up: first use of every kernel
+        # warm-up: first use of every kernel, and each stream's allocator pool
+        # and pinned staging grown by the whole frame pool arriving at once
+        for _ in range(2):
+            srv.run_trace([Frame(i, c, x) for i, (c, x) in enumerate(zip(pool, feats))], np.zeros(len(pool)))
+            srv.run_trace([Frame(i, c, x) for i, (c, x) in enumerate(zip(pool, feats))],
+                          np.arange(len(pool)) * 1e-3)
         torch.cuda.synchronize()
         for rate in [float(v) for v in args.rates.split(",")]:
             fr, arr = trace(rate, int(rate) + args.seed)
@@ -501,13 +509,14 @@ def run_serving(args):
             res.setdefault(name, {})[str(int(rate))] = st
             log(f"c5 {name} {rate:.0f} fps: p50 {st['p50_ms']:.2f} ms  p99 {st['p99_ms']:.2f} ms  "
                 f"{st['frames_per_s']:.1f} frames/s  mean batch {st['mean_batch']:.2f}")
+        srv.close()
         del srv
         torch.cuda.synchronize()
     top = str(int(float(args.rates.split(",")[-1])))
-    main = res["bf16_continuous"][top]
+    main = res[next(iter(res))][top]
     line = {
         "metric": "sparse-conv TFLOP/s + % MFMA peak per dtype vs plain FP32; serving p50/p99 ms",
-        "value": main["p99_ms"], "unit": "ms (p99 latency)", "n_gpus": 1, "steps": main["frames"], "warmup": 6,
+        "value": main["p99_ms"], "unit": "ms (p99 latency)", "n_gpus": 1, "steps": main["frames"], "warmup": 4 * 16,
         "ms_per_step": main["mean_ms"], "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "bf16", "data": "synthetic",
         "config": {"workload": "serving: Poisson arrivals, frames of 20k-200k voxels, 4-layer stack subm 16->32, "
@@ -673,7 +682,7 @@ def run_extra(args) -> dict:
         out["c4"] = {"error": repr(e)}
     log(f"extra c4: {time.perf_counter() - t0:.1f}s")
     a5 = copy.copy(args)
-    a5.seconds, a5.rates = 1.0, "20,50,100,200"
+    a5.seconds, a5.rates = 1.0, "20,50,100,200,400"
     t0 = time.perf_counter()
     try:
         l5 = run_serving(a5)

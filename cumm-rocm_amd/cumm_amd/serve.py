@@ -13,7 +13,11 @@ that arrive over time.  The server
 * runs each batch on its own HIP stream from a small pool: upload, device pair
   tables for every layer (cumm_amd.pairs), then one implicit_gemm2 launch per
   layer over all frames of the batch; features, tables and activations stay
-  on the device;
+  on the device.  Each stream has a launch thread of its own: the host work of
+  a batch (staging, table building, whose strided layers wait on their stream
+  for the output count, parameter set-up) blocks only that stream's thread,
+  so with two streams a batch formed while another is being launched or is
+  running starts at once and the two overlap on the device;
 * completes frames by polling the batch's event, releasing their memory
   reservation and slot, and records latency = completion - arrival.
 
@@ -26,6 +30,8 @@ from __future__ import annotations
 
 import collections
 import dataclasses
+import queue
+import threading
 import time
 from typing import Deque, Dict, List, Optional, Sequence
 
@@ -55,9 +61,11 @@ class Frame:
 class _Batch:
     frames: List[Frame]
     stream: torch.cuda.Stream
-    event: torch.cuda.Event
+    event: Optional[torch.cuda.Event]
     keep: list               # device objects alive until the batch completes
     reserved: int
+    launched: threading.Event = dataclasses.field(default_factory=threading.Event)
+    error: Optional[BaseException] = None
 
 
 class SparseConvServer:
@@ -88,6 +96,23 @@ class SparseConvServer:
         self.inflight = 0
         self.completed: List[Frame] = []
         self.t0 = time.perf_counter()
+        # one launch thread per stream (started on first use)
+        self._jobs: List["queue.Queue[Optional[_Batch]]"] = [queue.Queue() for _ in range(n_streams)]
+        self._workers: List[Optional[threading.Thread]] = [None] * n_streams
+
+    def close(self) -> None:
+        """Stops the launch threads (after the work they were given)."""
+        for si, w in enumerate(self._workers):
+            if w is not None:
+                self._jobs[si].put(None)
+                w.join()
+                self._workers[si] = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     # ------------------------------------------------------------ memory model
     def frame_bytes(self, n: int) -> int:
@@ -129,19 +154,45 @@ class SparseConvServer:
         return None
 
     def _launch(self, si: int) -> None:
+        """Forms a batch from the admitted frames and hands it to stream si's
+        launch thread."""
         frames = []
         while self.admitted and len(frames) < self.max_batch:
             frames.append(self.admitted.popleft())
-        s = self.streams[si]
         now = self.now()
-        coords = np.concatenate([f.coords for f in frames])
-        off = 0
-        for b, f in enumerate(frames):
-            n = f.coords.shape[0]
-            coords[off:off + n, 0] = b
-            off += n
+        for f in frames:
             f.start = now
             f.batch_size = len(frames)
+        b = _Batch(frames, self.streams[si], None, [], sum(self.frame_bytes(f.coords.shape[0]) for f in frames))
+        self.busy[si] = b
+        if self._workers[si] is None:
+            w = threading.Thread(target=self._worker, args=(si,), daemon=True, name=f"serve-launch-{si}")
+            self._workers[si] = w
+            w.start()
+        self._jobs[si].put(b)
+
+    def _worker(self, si: int) -> None:
+        while True:
+            b = self._jobs[si].get()
+            if b is None:
+                return
+            try:
+                self._enqueue(si, b)
+            except BaseException as e:  # reported by the scheduler thread
+                b.error = e
+            b.launched.set()
+
+    def _enqueue(self, si: int, b: _Batch) -> None:
+        """Host work and device launches of one batch (runs on the stream's
+        launch thread)."""
+        frames = b.frames
+        s = self.streams[si]
+        coords = np.concatenate([f.coords for f in frames])
+        off = 0
+        for k, f in enumerate(frames):
+            n = f.coords.shape[0]
+            coords[off:off + n, 0] = k
+            off += n
         feats = np.concatenate([f.feats for f in frames])
         pc_, px_ = self._staging(si, coords.shape[0], feats.shape[1])
         pc_[:coords.shape[0]].numpy()[...] = coords
@@ -155,22 +206,26 @@ class SparseConvServer:
             st.step()
             ev = torch.cuda.Event()
             ev.record(s)
-        self.busy[si] = _Batch(frames, s, ev, [c_dev, x_dev, tabs, st], sum(self.frame_bytes(f.coords.shape[0])
-                                                                             for f in frames))
+        b.keep = [c_dev, x_dev, tabs, st]
+        b.event = ev
 
     def _staging(self, si: int, n: int, c: int):
         """Pinned buffers of stream si holding at least n rows.  A buffer is
         refilled only once the stream's previous batch (whose upload read it)
         has completed, which _launch guarantees by taking idle streams only."""
         if self._pin_c[si] is None or self._pin_c[si].shape[0] < n or self._pin_x[si].shape[1] != c:
-            cap = max(n, 1 << 16)
+            cap = max(n + n // 2, 1 << 16)  # geometric growth
             self._pin_c[si] = torch.empty((cap, 4), dtype=torch.int32, pin_memory=True)
             self._pin_x[si] = torch.empty((cap, c), dtype=torch.float32, pin_memory=True)
         return self._pin_c[si], self._pin_x[si]
 
     def _poll(self) -> None:
         for si, b in self.busy.items():
-            if b is None or not b.event.query():
+            if b is None or not b.launched.is_set():
+                continue
+            if b.error is not None:
+                raise RuntimeError(f"launching a batch on stream {si} failed") from b.error
+            if not b.event.query():
                 continue
             t = self.now()
             st, tabs = b.keep[3], b.keep[2]

@@ -109,3 +109,37 @@ def test_poisson_trace_completes():
     assert st["frames"] == len(frames) and all(f.done >= f.arrival for f in frames)
     assert st["p99_ms"] >= st["p50_ms"] > 0
     print(st)
+
+
+def test_two_streams_overlap_and_match_alone():
+    """Two launch threads / streams: every batch of a trace arriving faster
+    than one stream drains runs on either stream, and each frame's output
+    still equals its batch-of-one result bit for bit."""
+    print("[serve] two streams", flush=True)
+    sizes = [1500, 400, 3000, 900, 2200, 1700, 2600, 800]
+    alone = []
+    s1 = _server(dt="bf16", max_batch=1)
+    for f in _frames(sizes, seed=21):
+        s1.submit(f)
+        s1.drain()
+        alone.append(f)
+    s1.close()
+    s2 = _server(dt="bf16", max_batch=3, n_streams=2)
+    used = set()
+    launch = s2._launch
+
+    def spy(si):
+        used.add(si)
+        launch(si)
+
+    s2._launch = spy
+    frames = _frames(sizes, seed=21)
+    arrivals = np.arange(len(frames)) * 2e-4
+    st = s2.run_trace(frames, arrivals)
+    s2.close()
+    torch.cuda.synchronize()
+    assert st["frames"] == len(frames)
+    assert used == {0, 1}
+    for a, m in zip(alone, sorted(frames, key=lambda f: f.fid)):
+        assert torch.equal(a.out_coords[:, 1:], m.out_coords[:, 1:])
+        assert torch.equal(a.out, m.out), f"frame {a.fid}"
