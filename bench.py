@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", choices=("c2", "c3", "c4", "c5", "f3"), default="c2",
+    ap.add_argument("--config", choices=("c2", "c3", "c4", "c5", "f3", "gemm"), default="c2",
                     help="c2: 120k voxels 64->64 (default); c3: precision form, 200k voxels 128->128; "
                          "c4: 32 clouds, strided 32->64 + subm 64->128, grouped launches under a HIP graph; "
                          "c5: serving, Poisson 20-200 frames/s, 4-layer BF16 stack; "
@@ -66,7 +66,7 @@ def parse():
                     help="c2 only: skip the precision, batched and serving forms in the same line")
     args = ap.parse_args()
     dflt = {"c2": (120000, 64, "f32,f16,bf16"), "c3": (200000, 128, "f32,bf16,f16,e4m3"),
-            "c4": (0, 0, "f16"), "c5": (0, 0, "bf16"), "f3": (120000, 64, "i8")}[args.config]
+            "c4": (0, 0, "f16"), "c5": (0, 0, "bf16"), "f3": (120000, 64, "i8"), "gemm": (0, 0, "f16")}[args.config]
     args.voxels = args.voxels or dflt[0]
     args.channels = args.channels or dflt[1]
     args.dtypes = args.dtypes or dflt[2]
@@ -630,6 +630,102 @@ def run_f3(args):
     print(json.dumps(line))
 
 
+def run_gemm(args, rows: int = 64000, n: int = 128, k: int = 128) -> dict:
+    """matmul2 in its spconv roles at rows = 64000, n = k = 128: ShuffleAC
+    (C[c_inds] = A[a_inds] @ B, gathered rows in, scattered rows out; B
+    row-major [k, n], read k-major in place, and n-major [n, k]) and ShuffleAB
+    (C = A[a_inds]^T @ B[b_inds], the weight-gradient form, default split) for
+    f32, f16, bf16 and int8 (int8 and int32 outputs).  Indices are random
+    permutations.  Per call: mean over `steps` back-to-back calls between HIP
+    events; algorithmic HBM bytes = gathered rows + written rows + B +
+    indices (ShuffleAB: both gathered operands + indices + the output)."""
+    import torch
+    from cumm_amd import tensorview as tv
+    from cumm_amd.gemm.main import GemmMainUnitTest
+    from cumm_amd.tensorview.gemm import GemmParams
+
+    rng = np.random.default_rng(args.seed)
+    stream = torch.cuda.current_stream()
+    tdt = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16, "s8": torch.int8,
+           "s32": torch.int32}
+    vdt = {"f32": tv.float32, "f16": tv.float16, "bf16": tv.bfloat16, "s8": tv.int8, "s32": tv.int32}
+    size = {"f32": 4, "f16": 2, "bf16": 2, "s8": 1, "s32": 4}
+    peak = {"f32": PEAK["f32"], "f16": PEAK["f16"], "bf16": PEAK["bf16"], "s8": 5.0e15}
+    descs = GemmMainUnitTest.get_all_algo_desp()
+
+    def desp(din, dout, shuffle, ta, tb):
+        for d in descs:
+            if (d.dtype_a == vdt[din] and d.dtype_c == vdt[dout] and int(d.shuffle_type) == shuffle
+                    and d.trans_a == ta and d.trans_b == tb and not d.trans_c and d.dacc != tv.float16):
+                return d
+        raise LookupError((din, dout, shuffle, ta, tb))
+
+    def operand(shape, dt):
+        if dt == "s8":
+            return torch.from_numpy(rng.integers(-128, 128, shape).astype(np.int8)).cuda()
+        return torch.from_numpy(rng.uniform(-1, 1, shape).astype(np.float32)).cuda().to(tdt[dt])
+
+    perm = lambda: torch.from_numpy(rng.permutation(rows).astype(np.int32)).cuda()
+    a_inds, c_inds, b_inds = perm(), perm(), perm()
+
+    def timed(p):
+        for _ in range(args.warmup):
+            GemmMainUnitTest.matmul2(p)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            GemmMainUnitTest.matmul2(p)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / args.steps * 1e-3
+
+    forms = {}
+    for din, dout in (("f32", "f32"), ("f16", "f16"), ("bf16", "bf16"), ("s8", "s8"), ("s8", "s32")):
+        es, eo = size[din], size[dout]
+        A = operand((rows, k), din)
+        flops = 2.0 * rows * n * k
+        for tb in (False, True):
+            B = operand((n, k) if tb else (k, n), din)
+            C = torch.zeros((rows, n), dtype=tdt[dout], device="cuda")
+            p = GemmParams()
+            p.algo_desp = desp(din, dout, 1, False, tb)
+            p.a, p.b, p.c = tv.from_torch(A), tv.from_torch(B), tv.from_torch(C)
+            p.a_inds, p.c_inds = tv.from_torch(a_inds), tv.from_torch(c_inds)
+            p.stream = stream.cuda_stream
+            t = timed(p)
+            byt = rows * k * es + rows * n * eo + k * n * es + 2 * rows * 4
+            forms[f"shuffle_ac_{din}_{dout}_{'nmajor_b' if tb else 'kmajor_b'}"] = {
+                "us": t * 1e6, "GB/s": byt / t / 1e9, "frac_hbm": byt / t / HBM_PEAK,
+                "TFLOP/s": flops / t / 1e12, "frac_mfma": flops / t / peak[din], "bytes": byt}
+            log(f"gemm ShuffleAC {din}->{dout} {'b [n,k]' if tb else 'b [k,n]'}: {t * 1e6:.1f} us  "
+                f"{byt / t / 1e9:.0f} GB/s  {flops / t / 1e12:.1f} TFLOP/s")
+        Bg = operand((rows, n), din)
+        C = torch.zeros((k, n), dtype=tdt[dout], device="cuda")
+        p = GemmParams()
+        p.algo_desp = desp(din, dout, 2, True, False)
+        p.a, p.b, p.c = tv.from_torch(A), tv.from_torch(Bg), tv.from_torch(C)
+        p.a_inds, p.b_inds = tv.from_torch(a_inds), tv.from_torch(b_inds)
+        p.stream = stream.cuda_stream
+        t = timed(p)
+        byt = rows * k * es + rows * n * es + 2 * rows * 4 + k * n * eo
+        forms[f"shuffle_ab_{din}_{dout}"] = {
+            "us": t * 1e6, "GB/s": byt / t / 1e9, "frac_hbm": byt / t / HBM_PEAK, "TFLOP/s": flops / t / 1e12,
+            "frac_mfma": flops / t / peak[din], "bytes": byt}
+        log(f"gemm ShuffleAB {din}->{dout}: {t * 1e6:.1f} us  {byt / t / 1e9:.0f} GB/s  {flops / t / 1e12:.1f} TFLOP/s")
+    main = forms["shuffle_ac_f16_f16_kmajor_b"]
+    return {
+        "metric": "matmul2 gather GEMM GB/s (ShuffleAC f16, rows 64000, n = k = 128)",
+        "value": main["GB/s"], "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": main["us"] * 1e-3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f16", "data": "synthetic",
+        "config": {"workload": f"matmul2 ShuffleAC / ShuffleAB, rows {rows}, n {n}, k {k}, random permutation indices"},
+        "roofline": {"bound": "hbm", "achieved": main["GB/s"], "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": main["frac_hbm"], "traffic": None},
+        "forms": forms,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -643,6 +739,9 @@ def main():
         return
     if args.config == "f3":
         return run_f3(args)
+    if args.config == "gemm":
+        print(json.dumps(run_gemm(args)))
+        return
     line = run_single(args, with_cpu=not args.no_cpu_baseline)
     if args.config == "c2" and not args.no_extra:
         line["configs"] = run_extra(args)
@@ -651,8 +750,9 @@ def main():
 
 def run_extra(args) -> dict:
     """The precision (200k voxels, 128 -> 128, f32 / bf16 / f16 / e4m3),
-    batched (32 clouds under a HIP graph) and serving (Poisson arrivals, 1 s
-    per rate) forms, summarised for the same output line as the c2 headline."""
+    batched (32 clouds under a HIP graph), matmul2 gather GEMM (rows 64000,
+    n = k = 128) and serving (Poisson arrivals, 1 s per rate) forms,
+    summarised for the same output line as the c2 headline."""
     import copy
 
     out = {}
@@ -681,6 +781,15 @@ def run_extra(args) -> dict:
     except Exception as e:
         out["c4"] = {"error": repr(e)}
     log(f"extra c4: {time.perf_counter() - t0:.1f}s")
+    t0 = time.perf_counter()
+    try:
+        ag = copy.copy(args)
+        ag.steps, ag.warmup = 50, 10
+        lg = run_gemm(ag)
+        out["gemm"] = {k: lg[k] for k in ("value", "unit", "ms_per_step", "config", "roofline", "forms")}
+    except Exception as e:
+        out["gemm"] = {"error": repr(e)}
+    log(f"extra gemm: {time.perf_counter() - t0:.1f}s")
     a5 = copy.copy(args)
     a5.seconds, a5.rates = 1.0, "20,50,100,200,400"
     t0 = time.perf_counter()

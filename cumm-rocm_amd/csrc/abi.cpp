@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <array>
 #include <climits>
+#include <cmath>
 #include <cstring>
 #include <sstream>
 #include <stdexcept>
@@ -422,6 +423,47 @@ const std::vector<cs_gemm_algo_desp>& gemm_catalogue() {
       d.shuffle_type = 2;
       v.push_back(d);
     }
+    // int8 operands with int32 accumulation and int32 epilogue arithmetic,
+    // int8 (saturated) or int32 output: "s8,s8,s8,s32,s32" and
+    // "s8,s8,s32,s32,s32" (cumm/gemm/main.py:348-366, 499-523)
+    for (int dc : {3, 1}) {
+      auto i8 = [&](cs_gemm_algo_desp d) {
+        d.dacc = 1;
+        d.dcomp = 1;
+        return d;
+      };
+      for (int ta = 0; ta < 2; ++ta)
+        for (int tb = 0; tb < 2; ++tb)
+          for (int tc = 0; tc < 2; ++tc) v.push_back(i8(rowgather_desp(3, 3, dc, rows, 64, ta, tb, tc)));
+      for (int tb = 0; tb < 2; ++tb) {
+        auto d = i8(rowgather_desp(3, 3, dc, rows, 64, 0, tb, 0));
+        d.shuffle_type = 1;
+        v.push_back(d);
+      }
+      auto d = i8(wgrad_desp(3, 3, dc, cs::kWgradRows));
+      d.shuffle_type = 2;
+      v.push_back(d);
+    }
+    // "f16,f16,f16,f16,f16" (f16 accumulator and epilogue type in the
+    // reference): the same kernels, which accumulate and compute in FP32 and
+    // round once into the f16 output.  No ShuffleAB entry: its split-K
+    // workspace, sized by the reference for f16 partial sums, cannot hold the
+    // FP32 partials (the f16 ShuffleAB entry with FP32 accumulation serves it)
+    {
+      auto h = [&](cs_gemm_algo_desp d) {
+        d.dacc = 7;
+        d.dcomp = 7;
+        return d;
+      };
+      for (int ta = 0; ta < 2; ++ta)
+        for (int tb = 0; tb < 2; ++tb)
+          for (int tc = 0; tc < 2; ++tc) v.push_back(h(rowgather_desp(7, 7, 7, rows, 64, ta, tb, tc)));
+      for (int tb = 0; tb < 2; ++tb) {
+        auto d = h(rowgather_desp(7, 7, 7, rows, 64, 0, tb, 0));
+        d.shuffle_type = 1;
+        v.push_back(d);
+      }
+    }
     return v;
   }();
   return cat;
@@ -804,6 +846,7 @@ struct GemmPlan {
   cs_tensor a, b, c, d;
   int64_t m = 0, n = 0, k = 0;
   int splits = 1;
+  bool b_kmajor = false;
   int64_t ws_a = 0, ws_b = 0, workspace = 0;
 };
 
@@ -886,6 +929,13 @@ GemmPlan plan_gemm(const cs_gemm_params& p) {
     check_index_dtype(p.c_inds, "c_inds");
     CS_RT(numel(p.c_inds) >= g.m, "c_inds must hold m entries");
   }
+  if (desp.dtype_a == 3) {
+    // int32 epilogue arithmetic: alpha, beta and act_alpha become int32 by
+    // truncation, and sigmoid has no integer form
+    CS_ARG(p.act_type != 2, "sigmoid is not available for int8 GEMMs (int32 compute)");
+    for (float v : {p.alpha, p.beta, p.act_alpha})
+      CS_ARG(std::isfinite(v) && std::fabs(v) < 2147483648.0f, "alpha, beta and act_alpha of an int8 GEMM must fit int32");
+  }
   const int es = dtype_bytes(p.a.dtype);
   const int vec = vec_elems(p.a.dtype);
   CS_ARG(aligned16(p.a.data) && aligned16(p.b.data), "a and b must be 16-byte aligned");
@@ -902,7 +952,10 @@ GemmPlan plan_gemm(const cs_gemm_params& p) {
     if (!trans_a) CS_ARG(row_stride(p.a) % vec == 0, "lda must be a multiple of ", vec);
     if (trans_b) CS_ARG(row_stride(p.b) % vec == 0, "ldb must be a multiple of ", vec);
     g.ws_a = trans_a ? align256(g.m * g.k * es) : 0;
-    g.ws_b = trans_b ? 0 : align256(g.k * g.n * es);
+    // a row-major [k, n] b is read k-major in place when its rows are whole
+    // 16-byte units; otherwise it is transposed into the workspace
+    g.b_kmajor = !trans_b && g.n % vec == 0 && row_stride(p.b) % vec == 0;
+    g.ws_b = trans_b || g.b_kmajor ? 0 : align256(g.k * g.n * es);
     g.workspace = g.ws_a + g.ws_b;
   }
   return g;
@@ -963,6 +1016,10 @@ void run_gemm(const cs_gemm_params& p) {
     r.beta = p.beta;
     r.act_alpha = p.act_alpha;
     r.act_type = p.act_type;
+    r.int_compute = p.a.dtype == 3 ? 1 : 0;
+    r.ialpha = (int32_t)p.alpha;
+    r.ibeta = (int32_t)p.beta;
+    r.iact_alpha = (int32_t)p.act_alpha;
     CS_HIP(cs::launch_reduce(p.c.dtype, r, s));
     return;
   }
@@ -976,43 +1033,41 @@ void run_gemm(const cs_gemm_params& p) {
   }
   const void* bptr = p.b.data;
   int64_t ldb = row_stride(p.b);
-  if (!g.tb) {  // b is [k][n] -> [n][k]
+  if (!g.tb && !g.b_kmajor) {  // b is [k][n] -> [n][k]
     void* bt = static_cast<char*>(ws.ptr) + g.ws_a;
     CS_HIP(cs::launch_transpose(p.b.data, bt, g.k, 1, g.n, row_stride(p.b), es, s));
     bptr = bt;
     ldb = g.k;
   }
-  cs::RowGatherArgs a;
+  cs::GemmRowsArgs a;
   std::memset(&a, 0, sizeof(a));
   a.a = aptr;
   a.lda = lda;
   a.a_bytes = (uint32_t)(g.ta ? g.m * g.k * es : span_bytes(p.a));
   a.b = bptr;
-  a.b_fs = 0;
-  a.b_ns = ldb;
-  a.b_bytes = (uint32_t)(g.tb ? span_bytes(p.b) : g.k * g.n * es);
+  a.ldb = ldb;
+  a.b_kmajor = g.b_kmajor ? 1 : 0;
+  a.b_bytes = (uint32_t)(g.tb || g.b_kmajor ? span_bytes(p.b) : g.k * g.n * es);
   a.c = p.c.data;
   a.ldc = row_stride(p.c);
   a.d = g.d.data;
   a.ldd = bias ? 0 : row_stride(g.d);
   a.d_is_bias = bias ? 1 : 0;
   if (g.shuffle == 1) {
-    a.pairs = empty(p.a_inds) ? nullptr : reinterpret_cast<const int32_t*>(p.a_inds.data);
-    a.pair_ld = 0;
-    a.out_row = reinterpret_cast<const int32_t*>(p.c_inds.data);
+    a.a_rows = empty(p.a_inds) ? nullptr : reinterpret_cast<const int32_t*>(p.a_inds.data);
+    a.c_rows = reinterpret_cast<const int32_t*>(p.c_inds.data);
   }
   a.m = (int32_t)g.m;
-  a.kred = (int32_t)g.k;
-  a.nout = (int32_t)g.n;
-  a.kv = 1;
-  a.mask_filter = 1u;
-  a.mw = 1;
+  a.nn = (int32_t)g.n;
+  a.k = (int32_t)g.k;
   a.alpha = p.alpha;
   a.beta = p.beta;
   a.act_alpha = p.act_alpha;
-  a.act_beta = p.act_beta;
   a.act_type = p.act_type;
-  CS_HIP(cs::launch_rowgather(p.a.dtype, p.c.dtype, g.kernel->tile_shape[1], a, s));
+  a.ialpha = (int32_t)p.alpha;
+  a.ibeta = (int32_t)p.beta;
+  a.iact_alpha = (int32_t)p.act_alpha;
+  CS_HIP(cs::launch_gemm_rows(p.a.dtype, p.c.dtype, a, s));
 }
 
 // ------------------------------------------------------------------ gather / scatter

@@ -212,6 +212,32 @@ __device__ __forceinline__ float apply_act(float v, int act, float act_alpha) {
   }
 }
 
+// Integer GEMM epilogue (LinearCombination with int32 accumulation and
+// compute, cumm/gemm/output_op/linear.py:112-140 with dtype_comp s32):
+// x = alpha * acc + beta * src in wrapping int32 arithmetic, then ReLU /
+// LeakyReLU (x * act_alpha) on int32; alpha, beta and act_alpha are the
+// caller's floats converted to int32 by truncation.  Sigmoid has no integer
+// form and is refused by the host.
+template <typename Args>
+__device__ __forceinline__ int32_t int_epilogue(const Args& p, int32_t acc, int32_t src) {
+  uint32_t x = (uint32_t)p.ialpha * (uint32_t)acc;
+  if (p.ibeta != 0) x += (uint32_t)p.ibeta * (uint32_t)src;
+  int32_t v = (int32_t)x;
+  if (p.act_type == kActReLU) v = v > 0 ? v : 0;
+  else if (p.act_type == kActLeakyReLU) v = v >= 0 ? v : (int32_t)((uint32_t)v * (uint32_t)p.iact_alpha);
+  return v;
+}
+// int32 -> output element: int8 saturates to [-128, 127] (the saturating
+// int32 -> int8 array conversion, include/tensorview/gemm/math/
+// numeric_convert.h:1091-1150); int32 is stored as is.
+template <typename OutT> __device__ __forceinline__ OutT int_out(int32_t v);
+template <> __device__ __forceinline__ int8_t int_out<int8_t>(int32_t v) {
+  return (int8_t)(v < -128 ? -128 : (v > 127 ? 127 : v));
+}
+template <> __device__ __forceinline__ int32_t int_out<int32_t>(int32_t v) { return v; }
+__device__ __forceinline__ int32_t to_i32(int8_t v) { return v; }
+__device__ __forceinline__ int32_t to_i32(int32_t v) { return v; }
+
 // int8 inference (Int8Inference output op, cumm/gemm/output_op/linear.py:198-392,
 // bias not pre-scaled): v = act(scale[n] * acc + bias[n] + beta * source[o][n]),
 // fused multiply-adds in float; source is output_add (or the output) row o.

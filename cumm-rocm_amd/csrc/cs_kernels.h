@@ -83,6 +83,9 @@ struct WgradArgs {
 // out[mm*o_ms + f*o_fs + nn] = act(alpha * tree_sum_b partial[f][b][mm][nn] + beta * src)
 // over the nb_f pieces of offset f (from total / fixed_total, as in the wgrad kernel);
 // src = src_is_bias ? src[f*N + nn] : src[mm*s_ms + f*s_fs + nn]  (src may alias out)
+// Integer form (int_compute, int8 operands): the partials hold int32 sums,
+// added in wrapping int32 arithmetic, and the epilogue is int_epilogue with
+// ialpha / ibeta / iact_alpha.
 struct ReduceArgs {
   const float* partial; const int32_t* total; int32_t fixed_total, max_blocks, min_rows, kv, M, N;
   void* out; int64_t o_ms, o_fs;
@@ -90,7 +93,33 @@ struct ReduceArgs {
   uint32_t mask_filter;
   float alpha, beta, act_alpha;
   int32_t act_type;
+  int32_t int_compute, ialpha, ibeta, iact_alpha;
 };
+
+// Gather GEMM (matmul2 NoShuffle / ShuffleAC):
+//   C[c_rows ? c_rows[i] : i][n] = epilogue(sum_k A[a_rows ? a_rows[i] : i][k] * B(n, k))
+// for 0 <= i < m, 0 <= n < nn, with B(n, k) = b[n*ldb + k] (n-major) or
+// b[k*ldb + n] (b_kmajor).  Float epilogue: act(alpha * acc + beta * src);
+// integer (int8 operands, int32 accumulation): int_epilogue, then int8
+// saturation or the int32 value.  src = d_is_bias ? d[n] : d[o*ldd + n].
+struct GemmRowsArgs {
+  const void* a; int64_t lda; uint32_t a_bytes;
+  const int32_t* a_rows;
+  const int32_t* c_rows;
+  const void* b; int64_t ldb; uint32_t b_bytes; int32_t b_kmajor;
+  void* c; int64_t ldc;
+  const void* d; int64_t ldd; int32_t d_is_bias;
+  int32_t m, nn, k;
+  float alpha, beta, act_alpha;
+  int32_t act_type;
+  int32_t ialpha, ibeta, iact_alpha;
+};
+hipError_t launch_gemm_rows(int in_dtype, int out_dtype, const GemmRowsArgs& args, hipStream_t s);
+bool gemm_rows_supported(int in_dtype, int out_dtype);
+hipError_t launch_gemm_rows_f32(int out_dtype, const GemmRowsArgs& args, hipStream_t s);
+hipError_t launch_gemm_rows_f16(int out_dtype, const GemmRowsArgs& args, hipStream_t s);
+hipError_t launch_gemm_rows_bf16(int out_dtype, const GemmRowsArgs& args, hipStream_t s);
+hipError_t launch_gemm_rows_i8(int out_dtype, const GemmRowsArgs& args, hipStream_t s);
 
 // block_n: output columns per wave, 64 or 128.
 hipError_t launch_rowgather_f32(int out_dtype, int block_n, const RowGatherArgs& args, hipStream_t s);

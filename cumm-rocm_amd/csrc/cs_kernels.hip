@@ -93,6 +93,10 @@ template <> struct WgFrag<e4m3> {
   }
 };
 
+// int8 rows have the byte layout of e4m3 rows; v_mfma_i32_16x16x64_i8 takes
+// its 64 reduction bytes per lane group in the order the fp8 pair takes them.
+template <> struct WgFrag<int8_t> : WgFrag<e4m3> {};
+
 // Stage geometry of the wgrad kernel for a TM x TN tile: R pairs per stage
 // (about 16 KB of rows, a multiple of the MFMA reduction chunk), CPA / CPB
 // 16-byte chunks per A / B row, LA / LB chunks per thread and stage.
@@ -396,12 +400,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void w
 // each element are then added in a fixed pairwise tree: the order depends
 // only on the piece count, never on scheduling.  V = 4 when the element count
 // of an offset is a multiple of 4 (vector loads stay inside one offset).
-template <typename OutT, int V>
+template <typename OutT, int V, bool INT>
 __global__ __launch_bounds__(256) void splitk_reduce_kernel(ReduceArgs p) {
   constexpr int PQ = 16;                     // piece lanes per element
   constexpr int PER = kWgradMaxBlocks / PQ;  // pieces per thread, at most
   // red[q][c * 16 + el]: element el * V + c of piece lane q.  The pitch puts
   // the four piece lanes of one wave on distinct banks (64 banks).
+  // INT: the partials are int32 sums, added as unsigned (wrapping) words;
+  // every value below is then a bit pattern in a float slot.
   constexpr int RP = V == 4 ? 16 * V + 16 : 16 * V;
   __shared__ float red[PQ][RP];
   const int64_t mn = (int64_t)p.M * p.N;
@@ -417,32 +423,62 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(ReduceArgs p) {
                                               (uint32_t)((int64_t)nb * mn * 4));
   float acc[V];
   if constexpr (V == 4) {
-    f32x4 v[PER];
+    if constexpr (INT) {
+      u32x4 v[PER];
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      const int b = q + PQ * k;
-      const u32x4 x = bld16(rp, b < nb ? (uint32_t)(((int64_t)b * mn + rem) * 4) : kBufOOB);
-      v[k] = f32x4{__uint_as_float(x.x), __uint_as_float(x.y), __uint_as_float(x.z), __uint_as_float(x.w)};
+      for (int k = 0; k < PER; ++k) {
+        const int b = q + PQ * k;
+        v[k] = bld16(rp, b < nb ? (uint32_t)(((int64_t)b * mn + rem) * 4) : kBufOOB);
+      }
+#pragma unroll
+      for (int w = PER / 2; w >= 1; w >>= 1)
+#pragma unroll
+        for (int i = 0; i < w; ++i) v[i] = v[2 * i] + v[2 * i + 1];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[c] = __uint_as_float(v[0][c]);
+    } else {
+      f32x4 v[PER];
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int b = q + PQ * k;
+        const u32x4 x = bld16(rp, b < nb ? (uint32_t)(((int64_t)b * mn + rem) * 4) : kBufOOB);
+        v[k] = f32x4{__uint_as_float(x.x), __uint_as_float(x.y), __uint_as_float(x.z), __uint_as_float(x.w)};
+      }
+#pragma unroll
+      for (int w = PER / 2; w >= 1; w >>= 1)
+#pragma unroll
+        for (int i = 0; i < w; ++i) v[i] = v[2 * i] + v[2 * i + 1];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[c] = v[0][c];
     }
-#pragma unroll
-    for (int w = PER / 2; w >= 1; w >>= 1)
-#pragma unroll
-      for (int i = 0; i < w; ++i) v[i] = v[2 * i] + v[2 * i + 1];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) acc[c] = v[0][c];
   } else {
-    float v[PER];
+    if constexpr (INT) {
+      uint32_t v[PER];
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      const int b = q + PQ * k;
-      v[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
-          rp, b < nb ? (uint32_t)(((int64_t)b * mn + rem) * 4) : kBufOOB, 0, 0));
+      for (int k = 0; k < PER; ++k) {
+        const int b = q + PQ * k;
+        v[k] = __builtin_amdgcn_raw_buffer_load_b32(rp, b < nb ? (uint32_t)(((int64_t)b * mn + rem) * 4) : kBufOOB,
+                                                    0, 0);
+      }
+#pragma unroll
+      for (int w = PER / 2; w >= 1; w >>= 1)
+#pragma unroll
+        for (int i = 0; i < w; ++i) v[i] = v[2 * i] + v[2 * i + 1];
+      acc[0] = __uint_as_float(v[0]);
+    } else {
+      float v[PER];
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int b = q + PQ * k;
+        v[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+            rp, b < nb ? (uint32_t)(((int64_t)b * mn + rem) * 4) : kBufOOB, 0, 0));
+      }
+#pragma unroll
+      for (int w = PER / 2; w >= 1; w >>= 1)
+#pragma unroll
+        for (int i = 0; i < w; ++i) v[i] = v[2 * i] + v[2 * i + 1];
+      acc[0] = v[0];
     }
-#pragma unroll
-    for (int w = PER / 2; w >= 1; w >>= 1)
-#pragma unroll
-      for (int i = 0; i < w; ++i) v[i] = v[2 * i] + v[2 * i + 1];
-    acc[0] = v[0];
   }
 #pragma unroll
   for (int c = 0; c < V; ++c) red[q][c * 16 + el] = acc[c];
@@ -453,27 +489,37 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(ReduceArgs p) {
   if (eo >= (int64_t)p.kv * mn) return;
   const int fo = (int)(eo / mn);
   if (!((p.mask_filter >> (fo & 31)) & 1u)) return;
-  float t[PQ];
-#pragma unroll
-  for (int i = 0; i < PQ; ++i) t[i] = red[i][(tid % V) * 16 + tid / V];
-#pragma unroll
-  for (int w = PQ / 2; w >= 1; w >>= 1)
-#pragma unroll
-    for (int i = 0; i < w; ++i) t[i] = t[2 * i] + t[2 * i + 1];
   const int64_t ro = eo - (int64_t)fo * mn;
   const int mm = (int)(ro / p.N);
   const int nn = (int)(ro - (int64_t)mm * p.N);
-  float r = p.alpha * t[0];
   const int64_t oi = (int64_t)mm * p.o_ms + (int64_t)fo * p.o_fs + nn;
-  if (p.beta != 0.f) {
-    const OutT* sp = reinterpret_cast<const OutT*>(p.src);
-    // a bias broadcasts over the GEMM rows and is indexed by the GEMM column,
-    // which for the [K, kv * C] view of the KRSC weight gradient is f * C + c
-    r += p.beta * to_f32(p.src_is_bias ? sp[(int64_t)fo * p.N + nn]
-                                       : sp[(int64_t)mm * p.s_ms + (int64_t)fo * p.s_fs + nn]);
+  const OutT* sp = reinterpret_cast<const OutT*>(p.src);
+  // a bias broadcasts over the GEMM rows and is indexed by the GEMM column,
+  // which for the [K, kv * C] view of the KRSC weight gradient is f * C + c
+  const int64_t si = p.src_is_bias ? (int64_t)fo * p.N + nn : (int64_t)mm * p.s_ms + (int64_t)fo * p.s_fs + nn;
+  if constexpr (INT) {
+    uint32_t t[PQ];
+#pragma unroll
+    for (int i = 0; i < PQ; ++i) t[i] = __float_as_uint(red[i][(tid % V) * 16 + tid / V]);
+#pragma unroll
+    for (int w = PQ / 2; w >= 1; w >>= 1)
+#pragma unroll
+      for (int i = 0; i < w; ++i) t[i] = t[2 * i] + t[2 * i + 1];
+    const int32_t src = p.ibeta != 0 ? to_i32(sp[si]) : 0;
+    reinterpret_cast<OutT*>(p.out)[oi] = int_out<OutT>(int_epilogue(p, (int32_t)t[0], src));
+  } else {
+    float t[PQ];
+#pragma unroll
+    for (int i = 0; i < PQ; ++i) t[i] = red[i][(tid % V) * 16 + tid / V];
+#pragma unroll
+    for (int w = PQ / 2; w >= 1; w >>= 1)
+#pragma unroll
+      for (int i = 0; i < w; ++i) t[i] = t[2 * i] + t[2 * i + 1];
+    float r = p.alpha * t[0];
+    if (p.beta != 0.f) r += p.beta * to_f32(sp[si]);
+    r = apply_act(r, p.act_type, p.act_alpha);
+    reinterpret_cast<OutT*>(p.out)[oi] = from_f32<OutT>(r);
   }
-  r = apply_act(r, p.act_type, p.act_alpha);
-  reinterpret_cast<OutT*>(p.out)[oi] = from_f32<OutT>(r);
 }
 
 // ============================================================ helpers
@@ -534,6 +580,26 @@ hipError_t launch_rowgather(int in_dtype, int out_dtype, int block_n, const RowG
   }
 }
 
+bool gemm_rows_supported(int in_dtype, int out_dtype) {
+  switch (in_dtype) {
+    case kF32: return out_dtype == kF32;
+    case kF16: return out_dtype == kF16;
+    case kBF16: return out_dtype == kBF16;
+    case kI8: return out_dtype == kI8 || out_dtype == kI32;
+    default: return false;
+  }
+}
+
+hipError_t launch_gemm_rows(int in_dtype, int out_dtype, const GemmRowsArgs& a, hipStream_t s) {
+  switch (in_dtype) {
+    case kF32: return launch_gemm_rows_f32(out_dtype, a, s);
+    case kF16: return launch_gemm_rows_f16(out_dtype, a, s);
+    case kBF16: return launch_gemm_rows_bf16(out_dtype, a, s);
+    case kI8: return launch_gemm_rows_i8(out_dtype, a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 template <typename T>
 static hipError_t launch_wgrad_t(int tm, int tn, dim3 grid, const WgradArgs& a, hipStream_t s) {
   dim3 block(256, 1, 1);
@@ -559,8 +625,14 @@ hipError_t launch_wgrad(int in_dtype, const WgradArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(wgrad_list_kernel, dim3(nblk, a.kv), dim3(256), 0, s, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
-  // 128-wide tiles where a dimension exceeds 64: each row is gathered once per tile
-  const int tm = a.M > 64 ? 128 : 64, tn = a.N > 64 ? 128 : 64;
+  // 128-wide tiles where a dimension exceeds 64: each row is gathered once per
+  // tile.  A single list (ShuffleAB) with few pieces keeps 64-wide tiles, so
+  // the grid has four times the blocks (rows read twice come from cache).
+  const int pieces = a.fixed_total >= 0
+                         ? std::min(a.max_blocks, std::max(1, (a.fixed_total + a.min_rows - 1) / a.min_rows))
+                         : a.max_blocks;
+  const bool few = a.fixed_total >= 0 && a.kv == 1 && pieces < 256;
+  const int tm = a.M > 64 && !few ? 128 : 64, tn = a.N > 64 && !few ? 128 : 64;
   const int tiles = ((a.M + tm - 1) / tm) * ((a.N + tn - 1) / tn);
   dim3 grid(tiles, a.kv, a.max_blocks);
   dim3 block(256, 1, 1);
@@ -570,28 +642,36 @@ hipError_t launch_wgrad(int in_dtype, const WgradArgs& a, hipStream_t s) {
     case kF16: return launch_wgrad_t<_Float16>(tm, tn, grid, a, s);
     case kBF16: return launch_wgrad_t<bf16>(tm, tn, grid, a, s);
     case kE4M3: return launch_wgrad_t<e4m3>(tm, tn, grid, a, s);
+    case kI8: return launch_wgrad_t<int8_t>(tm, tn, grid, a, s);
     default: return hipErrorInvalidValue;
   }
 }
 
-template <typename OutT>
+template <typename OutT, bool INT>
 static hipError_t launch_reduce_t(const ReduceArgs& a, hipStream_t s) {
   const int64_t mn = (int64_t)a.M * a.N;
   const int64_t total = (int64_t)a.kv * mn;
   if (total == 0) return hipSuccess;
   if (a.max_blocks < 1 || a.max_blocks > kWgradMaxBlocks) return hipErrorInvalidValue;
   if (mn % 4 == 0)
-    hipLaunchKernelGGL((splitk_reduce_kernel<OutT, 4>), dim3((unsigned)((total + 63) / 64)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((splitk_reduce_kernel<OutT, 4, INT>), dim3((unsigned)((total + 63) / 64)), dim3(256), 0, s, a);
   else
-    hipLaunchKernelGGL((splitk_reduce_kernel<OutT, 1>), dim3((unsigned)((total + 15) / 16)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((splitk_reduce_kernel<OutT, 1, INT>), dim3((unsigned)((total + 15) / 16)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
 hipError_t launch_reduce(int out_dtype, const ReduceArgs& a, hipStream_t s) {
+  if (a.int_compute) {
+    switch (out_dtype) {
+      case kI8: return launch_reduce_t<int8_t, true>(a, s);
+      case kI32: return launch_reduce_t<int32_t, true>(a, s);
+      default: return hipErrorInvalidValue;
+    }
+  }
   switch (out_dtype) {
-    case kF32: return launch_reduce_t<float>(a, s);
-    case kF16: return launch_reduce_t<_Float16>(a, s);
-    case kBF16: return launch_reduce_t<bf16>(a, s);
+    case kF32: return launch_reduce_t<float, false>(a, s);
+    case kF16: return launch_reduce_t<_Float16, false>(a, s);
+    case kBF16: return launch_reduce_t<bf16, false>(a, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -433,3 +433,86 @@ def test_golden_gemm_fixture():
     got = gemm_ref.matmul2(g["ab_a"], g["ab_b"], np.zeros(g["ab_c"].shape), trans_a=True, shuffle=2,
                            a_inds=g["ab_a_inds"], b_inds=g["ab_b_inds"])
     assert np.allclose(got, g["ab_c"], rtol=0, atol=1e-12)
+
+
+def _int_gemm_loops(A, B, alpha, beta, src, act, act_alpha, out_int8):
+    """Brute force: Python integers, explicit loops, wrap and saturate spelled out."""
+    def wrap(v):
+        v &= 0xFFFFFFFF
+        return v - (1 << 32) if v >= (1 << 31) else v
+    ia, ib, iact = int(alpha), int(beta), int(act_alpha)  # int() truncates toward zero
+    m, k = len(A), len(A[0])
+    n = len(B[0])
+    out = []
+    for i in range(m):
+        row = []
+        for j in range(n):
+            acc = 0
+            for t in range(k):
+                acc += int(A[i][t]) * int(B[t][j])
+            x = wrap(ia * wrap(acc))
+            if ib != 0:
+                x = wrap(x + wrap(ib * int(src[i][j])))
+            if act == 1 and x < 0:
+                x = 0
+            if act == 3 and x < 0:
+                x = wrap(x * iact)
+            if out_int8:
+                x = -128 if x < -128 else (127 if x > 127 else x)
+            row.append(x)
+        out.append(row)
+    return np.array(out, dtype=np.int64)
+
+
+@pytest.mark.parametrize("out_int8", [True, False])
+@pytest.mark.parametrize("act", [0, 1, 3])
+def test_int_gemm_oracle_brute_force(out_int8, act):
+    rng = np.random.default_rng(41 + act)
+    m, n, k = 5, 7, 48
+    A = rng.integers(-128, 128, (m, k))
+    B = rng.integers(-128, 128, (k, n))
+    src = rng.integers(-128, 128, (m, n)).astype(np.int8 if out_int8 else np.int32)
+    got = gemm_ref.matmul2_int(A, B, src, alpha=2.7, beta=-1.9, act_type=act, act_alpha=3.5)
+    exp = _int_gemm_loops(A.tolist(), B.tolist(), 2.7, -1.9, src.tolist(), act, 3.5, out_int8)
+    assert got.dtype == (np.int8 if out_int8 else np.int32)
+    assert np.array_equal(got.astype(np.int64), exp)
+
+
+def test_int_gemm_oracle_matches_float_form_without_saturation():
+    """Small operands: no value saturates or wraps, so the int32 result equals
+    the FP64 form exactly (alpha, beta integral)."""
+    rng = np.random.default_rng(3)
+    a = rng.integers(-3, 4, (40, 32))
+    b = rng.integers(-3, 4, (32, 24))
+    init = rng.integers(-5, 6, (40, 24)).astype(np.int32)
+    got = gemm_ref.matmul2_int(a, b, init, alpha=1.0, beta=1.0)
+    exp = gemm_ref.matmul2(a, b, init.astype(np.float64), alpha=1.0, beta=1.0)
+    assert np.array_equal(got.astype(np.float64), exp)
+    a_inds = rng.permutation(40)[:30]
+    c_inds = rng.permutation(40)[:30]
+    got = gemm_ref.matmul2_int(a, b, init.astype(np.int8), shuffle=1, a_inds=a_inds, c_inds=c_inds)
+    exp = gemm_ref.matmul2(a, b, init.astype(np.float64), shuffle=1, a_inds=a_inds, c_inds=c_inds)
+    assert np.array_equal(got.astype(np.float64), exp)
+    got = gemm_ref.matmul2_int(a, a, np.zeros((32, 32), np.int32), trans_a=True, shuffle=2,
+                               a_inds=a_inds, b_inds=c_inds)
+    exp = a[a_inds].T.astype(np.float64) @ a[c_inds]
+    assert np.array_equal(got.astype(np.float64), exp)
+
+
+def test_int_gemm_oracle_known_answers():
+    # saturation at both ends: 16 * 127 * 127 = 258064
+    a = np.full((2, 16), 127)
+    a[1] = -127
+    b = np.full((16, 1), 127)
+    assert gemm_ref.matmul2_int(a, b, np.zeros((2, 1), np.int8)).ravel().tolist() == [127, -128]
+    assert gemm_ref.matmul2_int(a, b, np.zeros((2, 1), np.int32)).ravel().tolist() == [258064, -258064]
+    # alpha and beta truncate toward zero: 1.9 -> 1, -0.5 -> 0 (the source is not read)
+    assert gemm_ref.matmul2_int(a, b, np.zeros((2, 1), np.int32), alpha=1.9).ravel().tolist() == [258064, -258064]
+    assert gemm_ref.matmul2_int(a, b, np.full((2, 1), 7, np.int32), alpha=-0.5, beta=0.5).ravel().tolist() == [0, 0]
+    # int32 wrap: 2^30 * 4 = 2^32 -> 0; LeakyReLU with act_alpha 2.9 -> 2
+    one = np.ones((1, 16))
+    zero = np.zeros((16, 1))
+    assert gemm_ref.matmul2_int(one, zero, np.full((1, 1), 4, np.int32), beta=float(1 << 30)).item() == 0
+    assert gemm_ref.matmul2_int(a[1:], b, np.zeros((1, 1), np.int32), act_type=3, act_alpha=2.9).item() == -516128
+    with pytest.raises(ValueError):
+        gemm_ref.matmul2_int(a, b, np.zeros((2, 1), np.int8), act_type=2)
