@@ -11,6 +11,7 @@
 #include <array>
 #include <climits>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <sstream>
 #include <stdexcept>
@@ -628,13 +629,19 @@ ConvPlan plan_conv(const cs_conv_params& p) {
   // Checks the generated kernels leave to the caller; here they guard memory safety.
   // Masks hold one 32-bit word per row, or div_up(kv, 32) words with dynamic
   // masks (MaskIGemmIteratorMaskLoaderDynamic, cumm/gemm/blockmma/mma.py:78-269).
+  // As in the reference kernels (cumm/conv/kernel.py:829, 935) the word count
+  // is div_up(kv, 32), derived from kv; ConvParams.mask_int_count is carried
+  // by the reference but not read by its kernels, so here it must be left at
+  // its default of 1 (or 0, a zero-filled struct) or agree with kv.
   if (desp.dynamic_mask) {
-    CS_ARG(kv <= 32 * cs::kMaxMaskWords, "dynamic masks support kernel volumes up to ",
-           32 * cs::kMaxMaskWords, ", got ", kv);
+    CS_ARG(kv <= 32 * cs::kMaxDynWords, "dynamic masks support kernel volumes up to ",
+           32 * cs::kMaxDynWords, ", got ", kv);
     plan.mw = (int)((kv + 31) / 32);
   } else {
     CS_ARG(kv <= 32, "kernel volume > 32 needs dynamic (multi-word) masks");
   }
+  CS_ARG(p.mask_int_count <= 1 || p.mask_int_count == plan.mw, "mask_int_count ", p.mask_int_count,
+         " does not match the kernel volume ", kv, " (", plan.mw, " mask words per row)");
   const int dt_in = p.input.dtype, dt_w = p.weight.dtype, dt_out = p.output.dtype;
   const int abc[3] = {desp.gemm.dtype_a, desp.gemm.dtype_b, desp.gemm.dtype_c};
   // conv_iwo_012_to_abc (params.h:225-237): i/w/o -> a/b/c
@@ -1165,6 +1172,36 @@ int copy_params(const std::vector<GsParam>& v, int32_t* out, int cap) {
 // ====================================================================== allocation-free conv
 // Used by cs_implicit_gemm2_noalloc (cs_noalloc.hip), which owns the memory.
 namespace cs_detail {
+// ConvParams.verbose (cumm/conv/main.py:704-729): time the call's launches on
+// its stream (synchronising, as the reference's timer does) and print the
+// kernel name, the main kernel's register count and the time in ms.
+template <typename F>
+void run_verbose(const cs_conv_params& p, F&& body) {
+  hipStream_t s = reinterpret_cast<hipStream_t>(p.stream);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  CS_HIP(hipEventCreate(&e0));
+  CS_HIP(hipEventCreate(&e1));
+  struct Free {
+    hipEvent_t a, b;
+    ~Free() {
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+    }
+  } fr{e0, e1};
+  cs::last_kernel() = nullptr;
+  CS_HIP(hipEventRecord(e0, s));
+  body();
+  CS_HIP(hipEventRecord(e1, s));
+  CS_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  CS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  int regs = -1;
+  hipFuncAttributes attr;
+  if (cs::last_kernel() != nullptr && hipFuncGetAttributes(&attr, cs::last_kernel()) == hipSuccess) regs = attr.numRegs;
+  std::printf("%s kernel num regs: %d time: %f\n", conv_repr(p.conv_algo_desp).c_str(), regs, (double)ms);
+  std::fflush(stdout);
+}
+
 
 __attribute__((visibility("hidden"))) int fail_runtime(const char* msg) {
   g_last_error = msg;
@@ -1198,7 +1235,8 @@ __attribute__((visibility("hidden"))) int implicit_gemm2_in(const cs_conv_params
     q.workspace.shape[0] = cap;
     q.workspace.stride[0] = 1;
     q.workspace.device = device;
-    run_conv(q);
+    if (q.verbose) run_verbose(q, [&] { run_conv(q); });
+    else run_conv(q);
   });
 }
 
@@ -1212,7 +1250,8 @@ int cs_abi_version(void) { return CS_ABI_VERSION; }
 int cs_implicit_gemm2(const cs_conv_params* params) {
   return guarded([&] {
     CS_ARG(params != nullptr, "params is null");
-    run_conv(*params);
+    if (params->verbose) cs_detail::run_verbose(*params, [&] { run_conv(*params); });
+    else run_conv(*params);
   });
 }
 

@@ -199,6 +199,27 @@ __device__ __forceinline__ void reverse_kv(uint32_t (&m)[MW], int kv) {
   }
 }
 
+// Word w, in kernel order, of a kv-offset mask whose original words are
+// word(q) (zero outside [0, div_up(kv, 32)), bits at or past kv clear):
+// forward, word(w); reversed (dgrad of a submanifold layer), bit f is
+// original bit kv - 1 - f.  For kv <= 32 this is reverse_kv of one word.
+template <typename F>
+__device__ __forceinline__ uint32_t kernel_word(const F& word, int w, int kv, bool rev) {
+  if (!rev) return word(w);
+  const int hi = kv - 1 - 32 * w;  // original bit of kernel bit 32w
+  if (hi < 0) return 0u;
+  const int lo = hi - 31;          // original bit of kernel bit 32w + 31
+  uint32_t x;                      // bit j: original bit lo + j (zero below bit 0)
+  if (lo >= 0) {
+    const int q = lo >> 5, sh = lo & 31;
+    x = word(q) >> sh;
+    if (sh) x |= word(q + 1) << (32 - sh);
+  } else {
+    x = word(0) << (-lo);
+  }
+  return __builtin_bitreverse32(x);
+}
+
 // ---------------------------------------------------------------- epilogue
 // D = act(alpha * acc + beta * source); act codes follow tv::gemm::Activation.
 enum : int { kActNone = 0, kActReLU = 1, kActSigmoid = 2, kActLeakyReLU = 3 };

@@ -362,7 +362,7 @@ static hipError_t launch_gm_kernel(const GemmRowsArgs& a, hipStream_t s) {
   const int ngroups = (a.m + kGmRows - 1) / kGmRows;
   const int per_slice = std::max(1, std::min(ngroups, per_cu * ncu / std::max(slices, 1)));
   if (ngroups == 0 || slices == 0) return hipSuccess;
-  hipLaunchKernelGGL((gemm_rows_kernel<T, OutT, NB, KC, KMAJ, MULTI>), dim3((unsigned)per_slice, (unsigned)slices, 1),
+  CS_LAUNCH((gemm_rows_kernel<T, OutT, NB, KC, KMAJ, MULTI>), dim3((unsigned)per_slice, (unsigned)slices, 1),
                      dim3(64 * kGmWaves, 1, 1), 0, s, a);
   return hipGetLastError();
 }

@@ -33,7 +33,7 @@ struct RowGatherArgs {
   const int32_t* out_row;
   const uint32_t* mask;      // sorted per-row masks ([m][mw] words); null -> every offset of the filter
   uint32_t* mask_out;        // per-16-row tile masks ([tiles][mw], forward), may be null
-  int32_t mw;                // mask words per row: 1 (kv <= 32) or div_up(kv, 32) <= 4 (dynamic masks)
+  int32_t mw;                // mask words per row: 1 (kv <= 32) or div_up(kv, 32) <= kMaxDynWords (dynamic masks)
   const float* scale;        // per-output-column multiplier, may be null
   // int8 inference (int8 a, b, c): c = act(scale[n] * acc + bias_f[n] + beta * d[o][n])
   const float* bias_f;
@@ -145,10 +145,20 @@ hipError_t launch_gather_rows(const void* in, void* out, const int32_t* idx, int
 hipError_t launch_scatter_add_f32(const float* in, float* out, const int32_t* idx, int64_t m, int64_t n_out,
                                   int64_t cols, int64_t in_ld, int64_t out_ld, hipStream_t s);
 
+// The main kernel of this thread's last launch (a verbose caller reports its
+// register count).  CS_LAUNCH records it and launches.
+const void*& last_kernel();
+#define CS_LAUNCH(K, ...)                                          \
+  do {                                                             \
+    ::cs::last_kernel() = reinterpret_cast<const void*>(&K);       \
+    hipLaunchKernelGGL(K, __VA_ARGS__);                            \
+  } while (0)
+
 constexpr int kRowsPerWave = 64;     // four 16-row mask tiles per wave
 constexpr int kWavesPerBlock = 4;
 constexpr int kMaskTile = 16;        // rows per kernel-offset mask tile
-constexpr int kMaxMaskWords = 4;     // dynamic masks: kv <= 128
+constexpr int kMaxMaskWords = 4;     // dynamic masks: words per window of offsets
+constexpr int kMaxDynWords = 32;     // dynamic masks: words per row, kv <= 1024
 constexpr int kWgradRows = 64;       // rows staged per wgrad step
 constexpr int kWgradMaxBlocks = 128; // pieces per offset, upper bound
 constexpr int kWgradTile = 64;       // wgrad output tile (M and N)

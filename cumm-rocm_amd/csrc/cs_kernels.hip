@@ -135,15 +135,21 @@ __global__ __launch_bounds__(256) void wgrad_rowmask_kernel(WgradArgs p) {
   const __amdgpu_buffer_rsrc_t rm =
       make_rsrc(p.mask, p.mask != nullptr ? (uint32_t)p.num_mask * (uint32_t)p.mw * 4u : 0u);
   const int i = p.a_rows[r];
-  uint32_t mv[kMaxMaskWords];
+  // words in groups of four (dynamic masks carry up to kMaxDynWords)
+  for (int w0 = 0; w0 < p.mw; w0 += 4) {
+    uint32_t mv[4];
 #pragma unroll
-  for (int w = 0; w < kMaxMaskWords; ++w)
-    mv[w] = __builtin_amdgcn_raw_buffer_load_b32(
-        rm, w < p.mw && mi < p.num_mask ? ((uint32_t)mi * (uint32_t)p.mw + (uint32_t)w) * 4u : kBufOOB, 0, 0);
-  if ((unsigned)i >= (unsigned)p.nrows) return;
+    for (int j = 0; j < 4; ++j) {
+      const int w = w0 + j;
+      mv[j] = __builtin_amdgcn_raw_buffer_load_b32(
+          rm, w < p.mw && mi < p.num_mask ? ((uint32_t)mi * (uint32_t)p.mw + (uint32_t)w) * 4u : kBufOOB, 0, 0);
+    }
+    if ((unsigned)i >= (unsigned)p.nrows) return;
 #pragma unroll
-  for (int w = 0; w < kMaxMaskWords; ++w)
-    if (w < p.mw) p.rowmask[(int64_t)i * p.mw + w] = p.mask_filter & (p.mask != nullptr ? mv[w] : 0xffffffffu);
+    for (int j = 0; j < 4; ++j)
+      if (w0 + j < p.mw)
+        p.rowmask[(int64_t)i * p.mw + w0 + j] = p.mask_filter & (p.mask != nullptr ? mv[j] : 0xffffffffu);
+  }
 }
 
 // Pair-list construction, two passes over [kv] x [blocks of kWgradListRows
@@ -580,6 +586,11 @@ hipError_t launch_rowgather(int in_dtype, int out_dtype, int block_n, const RowG
   }
 }
 
+const void*& last_kernel() {
+  static thread_local const void* k = nullptr;
+  return k;
+}
+
 bool gemm_rows_supported(int in_dtype, int out_dtype) {
   switch (in_dtype) {
     case kF32: return out_dtype == kF32;
@@ -603,10 +614,10 @@ hipError_t launch_gemm_rows(int in_dtype, int out_dtype, const GemmRowsArgs& a, 
 template <typename T>
 static hipError_t launch_wgrad_t(int tm, int tn, dim3 grid, const WgradArgs& a, hipStream_t s) {
   dim3 block(256, 1, 1);
-  if (tm == 64 && tn == 64) hipLaunchKernelGGL((wgrad_kernel<T, 64, 64>), grid, block, 0, s, a);
-  else if (tm == 64) hipLaunchKernelGGL((wgrad_kernel<T, 64, 128>), grid, block, 0, s, a);
-  else if (tn == 64) hipLaunchKernelGGL((wgrad_kernel<T, 128, 64>), grid, block, 0, s, a);
-  else hipLaunchKernelGGL((wgrad_kernel<T, 128, 128>), grid, block, 0, s, a);
+  if (tm == 64 && tn == 64) CS_LAUNCH((wgrad_kernel<T, 64, 64>), grid, block, 0, s, a);
+  else if (tm == 64) CS_LAUNCH((wgrad_kernel<T, 64, 128>), grid, block, 0, s, a);
+  else if (tn == 64) CS_LAUNCH((wgrad_kernel<T, 128, 64>), grid, block, 0, s, a);
+  else CS_LAUNCH((wgrad_kernel<T, 128, 128>), grid, block, 0, s, a);
   return hipGetLastError();
 }
 

@@ -35,7 +35,8 @@ constexpr int kRgRedPitch = 68;   // LDS floats per row of a partial tile
 // HAS_PAIRS: gather through a pair table (else the identity / a_inds-free GEMM).
 // FULL: kred is a multiple of the per-pass reduction width and nout a multiple
 // of 64, so no lane needs a reduction- or column-range predicate.
-// MW: mask words handled (1, or kMaxMaskWords for dynamic masks, kv > 32).
+// MW: mask words per window of offsets (1, or kMaxMaskWords for dynamic
+// masks, kv > 32, walked window by window).
 template <typename T, typename OutT, bool HAS_PAIRS, bool FULL, int MW>
 __global__ __launch_bounds__(256) void rowgather_kernel(RowGatherArgs p) {
   using MM = Mma<T>;
@@ -44,6 +45,7 @@ __global__ __launch_bounds__(256) void rowgather_kernel(RowGatherArgs p) {
   constexpr int KCH = RgChunks<T>::value;
   constexpr int NB = 4;
   __shared__ float red[4][kRgRows][kRgRedPitch];
+  __shared__ uint32_t tmw[MW > 1 ? 4 : 1][MW > 1 ? 4 : 1][MW > 1 ? kMaxDynWords : 1];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so give
@@ -68,46 +70,59 @@ __global__ __launch_bounds__(256) void rowgather_kernel(RowGatherArgs p) {
     mycol = p.pair_col ? p.pair_col[myrow] : myrow;
     myout = p.out_row ? p.out_row[myrow] : myrow;
   }
-  // OR of the 16 row masks of each tile (lanes 16s..16s+15 hold tile s), word by word.
+  // OR of the 16 row masks of each tile (lanes 16s..16s+15 hold tile s)
   uint32_t sm[4][MW];
-#pragma unroll
-  for (int w = 0; w < MW; ++w) {
+  uint32_t live = 0;
+  if constexpr (MW == 1) {
     uint32_t g = 0u;
-    if (rvalid && w < p.mw) g = p.mask ? p.mask[(int64_t)myrow * p.mw + w] : 0xffffffffu;
+    if (rvalid) g = p.mask ? p.mask[myrow] : 0xffffffffu;
     g |= __shfl_xor(g, 1);
     g |= __shfl_xor(g, 2);
     g |= __shfl_xor(g, 4);
     g |= __shfl_xor(g, 8);
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
-      sm[s][w] = __builtin_amdgcn_readlane(g, 16 * s) & p.mask_filter & kv_word(p.kv, w);
-    if (wave == 0 && p.mask_out != nullptr && lane < 4 && row0 + 16 * lane < p.m && w < p.mw) {
-      const uint32_t v = lane == 0 ? sm[0][w] : lane == 1 ? sm[1][w] : lane == 2 ? sm[2][w] : sm[3][w];
-      p.mask_out[(int64_t)(row0 / kMaskTile + lane) * p.mw + w] = v;
+    for (int s = 0; s < 4; ++s) sm[s][0] = __builtin_amdgcn_readlane(g, 16 * s) & p.mask_filter & kv_word(p.kv, 0);
+    if (wave == 0 && p.mask_out != nullptr && lane < 4 && row0 + 16 * lane < p.m) {
+      const uint32_t v = lane == 0 ? sm[0][0] : lane == 1 ? sm[1][0] : lane == 2 ? sm[2][0] : sm[3][0];
+      p.mask_out[row0 / kMaskTile + lane] = v;
     }
-  }
-  uint32_t live = 0;
 #pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    uint32_t any = 0u;
+    for (int s = 0; s < 4; ++s)
+      if (row0 + 16 * s < p.m && (!p.skip_empty || sm[s][0] != 0u)) live |= 1u << s;
+    if (p.reverse_mask) {
 #pragma unroll
-    for (int w = 0; w < MW; ++w) any |= sm[s][w];
-    if (row0 + 16 * s < p.m && (!p.skip_empty || any != 0u)) live |= 1u << s;
-  }
-  if (p.reverse_mask) {
+      for (int s = 0; s < 4; ++s) reverse_kv(sm[s], p.kv);
+    }
+  } else {
+    // dynamic masks: the four tiles' OR words, all of them, in this wave's
+    // LDS area (every wave holds all 64 rows, so no barrier is needed); the
+    // offsets are then walked in windows of MW words, in kernel order
+    for (int q0 = 0; q0 < p.mw; q0 += 8) {
+      uint32_t g[8];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) reverse_kv(sm[s], p.kv);
-  }
-  // This wave's share: offsets f with f % 4 == wave.  The split depends on the
-  // offset index only, never on which offsets the tile's union holds, so every
-  // row's partial sums (and hence its bits) are the same whatever rows share
-  // its tile: a cloud gives identical rows alone and inside a batch.
-  uint32_t mine[MW];
-  uint32_t mine_any = 0u;
+      for (int j = 0; j < 8; ++j) {
+        g[j] = 0u;
+        if (rvalid && q0 + j < p.mw) g[j] = p.mask ? p.mask[(int64_t)myrow * p.mw + q0 + j] : 0xffffffffu;
+      }
 #pragma unroll
-  for (int w = 0; w < MW; ++w) {
-    mine[w] = (sm[0][w] | sm[1][w] | sm[2][w] | sm[3][w]) & (0x11111111u << wave);
-    mine_any |= mine[w];
+      for (int j = 0; j < 8; ++j) {
+        uint32_t v = g[j];
+        v |= __shfl_xor(v, 1);
+        v |= __shfl_xor(v, 2);
+        v |= __shfl_xor(v, 4);
+        v |= __shfl_xor(v, 8);
+        if (lr == 0 && q0 + j < p.mw) tmw[wave][lg][q0 + j] = v & p.mask_filter & kv_word(p.kv, q0 + j);
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      uint32_t any = 0u;
+      for (int q = 0; q < p.mw; ++q) any |= tmw[wave][s][q];
+      if (row0 + 16 * s < p.m && (!p.skip_empty || __builtin_amdgcn_readfirstlane(any) != 0u)) live |= 1u << s;
+      if (wave == 0 && p.mask_out != nullptr && row0 + 16 * s < p.m)
+        for (int q = lane; q < p.mw; q += 64) p.mask_out[(int64_t)(row0 / kMaskTile + s) * p.mw + q] = tmw[0][s][q];
+    }
   }
 
   int cols[4];
@@ -122,9 +137,10 @@ __global__ __launch_bounds__(256) void rowgather_kernel(RowGatherArgs p) {
 
   // pair indices of offset f for the rows this lane feeds (-1: no neighbour).
   // The loaded value is not touched here, so the load stays in flight.
+  int base = 0;  // first offset of the current window (dynamic masks)
   auto load_idx = [&](int f, int (&idx)[4]) {
 #pragma unroll
-    for (int s = 0; s < 4; ++s) idx[s] = HAS_PAIRS ? p.pairs[(int64_t)f * p.pair_ld + cols[s]] : cols[s];
+    for (int s = 0; s < 4; ++s) idx[s] = HAS_PAIRS ? p.pairs[(int64_t)(base + f) * p.pair_ld + cols[s]] : cols[s];
   };
   // operand fragments of offset f (f < 0: none) for reduction block kb; rows
   // with index -1, tiles without f, columns past nout and reduction elements
@@ -133,7 +149,7 @@ __global__ __launch_bounds__(256) void rowgather_kernel(RowGatherArgs p) {
   const __amdgpu_buffer_rsrc_t rb = make_rsrc(p.b, p.b_bytes);
   auto load_frags = [&](int f, const int (&idx)[4], int kb, u32x4 (&af)[4][KCH], u32x4 (&bfr)[KCH][NB]) {
     const bool fvalid = f >= 0;
-    const int64_t fb = (int64_t)(fvalid ? f : 0) * p.b_fs;
+    const int64_t fb = (int64_t)(fvalid ? base + f : 0) * p.b_fs;
 #pragma unroll
     for (int kc = 0; kc < KCH; ++kc) {
       const int k = kb + kc * CH + lg * E;
@@ -167,6 +183,18 @@ __global__ __launch_bounds__(256) void rowgather_kernel(RowGatherArgs p) {
 
   // This wave's offsets, two at a time: a pair's fragment loads form one
   // memory round trip, and the next pair's indices load while the MFMAs run.
+  auto window = [&]() {
+  // This wave's share: offsets f with f % 4 == wave.  The split depends on the
+  // offset index only, never on which offsets the tile's union holds, so every
+  // row's partial sums (and hence its bits) are the same whatever rows share
+  // its tile: a cloud gives identical rows alone and inside a batch.
+  uint32_t mine[MW];
+  uint32_t mine_any = 0u;
+#pragma unroll
+  for (int w = 0; w < MW; ++w) {
+    mine[w] = (sm[0][w] | sm[1][w] | sm[2][w] | sm[3][w]) & (0x11111111u << wave);
+    mine_any |= mine[w];
+  }
   for (int kb = 0; kb < p.kred; kb += KCH * CH) {
     if (!mine_any) break;
     uint32_t rem[MW];
@@ -203,6 +231,22 @@ __global__ __launch_bounds__(256) void rowgather_kernel(RowGatherArgs p) {
       if (fc < 0) break;
       fa = fc;
       fb = fd;
+    }
+  }
+  };
+  if constexpr (MW == 1) {
+    window();
+  } else {
+    const bool rev = p.reverse_mask != 0;
+    for (int win = 0; 32 * MW * win < p.kv; ++win) {
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        auto word = [&](int q) -> uint32_t { return q >= 0 && q < p.mw ? tmw[wave][s][q] : 0u; };
+#pragma unroll
+        for (int i = 0; i < MW; ++i) sm[s][i] = __builtin_amdgcn_readfirstlane(kernel_word(word, MW * win + i, p.kv, rev));
+      }
+      base = 32 * MW * win;
+      window();
     }
   }
 
@@ -270,11 +314,11 @@ static hipError_t launch_block(const RowGatherArgs& a, hipStream_t s) {
   constexpr int pass = RgChunks<T>::value * Mma<T>::kChunk;
   const bool full = a.kred % pass == 0 && a.nout % 64 == 0;
   if (a.pairs) {
-    if (full) hipLaunchKernelGGL((rowgather_kernel<T, OutT, true, true, MW>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((rowgather_kernel<T, OutT, true, false, MW>), grid, block, 0, s, a);
+    if (full) CS_LAUNCH((rowgather_kernel<T, OutT, true, true, MW>), grid, block, 0, s, a);
+    else CS_LAUNCH((rowgather_kernel<T, OutT, true, false, MW>), grid, block, 0, s, a);
   } else if constexpr (MW == 1) {
-    if (full) hipLaunchKernelGGL((rowgather_kernel<T, OutT, false, true, 1>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((rowgather_kernel<T, OutT, false, false, 1>), grid, block, 0, s, a);
+    if (full) CS_LAUNCH((rowgather_kernel<T, OutT, false, true, 1>), grid, block, 0, s, a);
+    else CS_LAUNCH((rowgather_kernel<T, OutT, false, false, 1>), grid, block, 0, s, a);
   } else {
     return hipErrorInvalidValue;  // multi-word masks come with a pair table
   }
@@ -303,7 +347,7 @@ static hipError_t launch_transposed(const RowGatherArgs& a, int block_n, hipStre
 template <typename T, typename OutT>
 static hipError_t launch_rg(const RowGatherArgs& a, int block_n, hipStream_t s) {
   if (block_n != 16 && block_n != 32 && block_n != 64 && block_n != 128) return hipErrorInvalidValue;
-  if (a.mw < 1 || a.mw > kMaxMaskWords) return hipErrorInvalidValue;
+  if (a.mw < 1 || a.mw > kMaxDynWords) return hipErrorInvalidValue;
   bool used = false;
   hipError_t e;
   if (a.mw > 1) {

@@ -24,7 +24,9 @@ constexpr int kStWaves = 8;                 // waves per block = mask tiles per 
 constexpr int kStLdsBytes = 2 * 32768;      // two weight stages of up to 32 KiB
 
 // KC: reduction chunks (kred / chunk); NB: 16-column blocks (nout <= 16 NB);
-// MW: mask words handled (1, or kMaxMaskWords for dynamic masks, kv > 32)
+// MW: mask words per window of offsets (1, or kMaxMaskWords for dynamic
+// masks, kv > 32: offsets are walked window by window, each window's words in
+// kernel order; a row's sum still runs over its offsets in ascending order)
 template <typename T, typename OutT, int KC, int NB, int MW>
 __global__ __launch_bounds__(64 * kStWaves) void rowgather_stream_kernel(RowGatherArgs p, int ntiles) {
   using MM = Mma<T>;
@@ -39,7 +41,8 @@ __global__ __launch_bounds__(64 * kStWaves) void rowgather_stream_kernel(RowGath
   constexpr int NT = 64 * kStWaves;
   constexpr int WCH = (NCOL * CHUNKS + NT - 1) / NT;  // 16-byte weight chunks per thread
   __shared__ __attribute__((aligned(16))) char wlds[kStLdsBytes];
-  __shared__ uint32_t wmask[kStWaves][MW];
+  __shared__ uint32_t wmask[kStWaves][1];
+  __shared__ uint32_t tmask[MW > 1 ? kStWaves : 1][MW > 1 ? kMaxDynWords : 1];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -50,46 +53,58 @@ __global__ __launch_bounds__(64 * kStWaves) void rowgather_stream_kernel(RowGath
   const int r = row0 + lr;
   const bool rvalid = t < ntiles && r < p.m;
 
-  // ---- this wave's tile: masks (word by word), pair column, output row of row lr
+  // ---- this wave's tile: pair column, output row of row lr
   int col = 0, orow = 0;
   if (rvalid) {
     col = p.pair_col ? p.pair_col[r] : r;
     orow = p.out_row ? p.out_row[r] : r;
   }
   uint32_t km[MW];
-  uint32_t any = 0u;
-#pragma unroll
-  for (int w = 0; w < MW; ++w) {
+  uint32_t U[MW];
+  bool live;
+  if constexpr (MW == 1) {
     uint32_t m = 0u;
-    if (rvalid && w < p.mw) m = p.mask ? p.mask[(int64_t)r * p.mw + w] : 0xffffffffu;
+    if (rvalid) m = p.mask ? p.mask[r] : 0xffffffffu;
     m |= __shfl_xor(m, 1);
     m |= __shfl_xor(m, 2);
     m |= __shfl_xor(m, 4);
     m |= __shfl_xor(m, 8);
-    km[w] = __builtin_amdgcn_readfirstlane(m) & p.mask_filter & kv_word(p.kv, w);
-    if (t < ntiles && w < p.mw && p.mask_out != nullptr && lane == 0) p.mask_out[(int64_t)t * p.mw + w] = km[w];
-    any |= km[w];
-  }
-  const bool live = t < ntiles && !(p.skip_empty && any == 0u);
-  if (!live) {
-#pragma unroll
-    for (int w = 0; w < MW; ++w) km[w] = 0u;
-  }
-  if (p.reverse_mask && any) reverse_kv(km, p.kv);
-  if (lane == 0) {
-#pragma unroll
-    for (int w = 0; w < MW; ++w) wmask[wave][w] = km[w];
-  }
-  __syncthreads();
-  uint32_t U[MW];
-#pragma unroll
-  for (int w = 0; w < MW; ++w) {
+    km[0] = __builtin_amdgcn_readfirstlane(m) & p.mask_filter & kv_word(p.kv, 0);
+    if (t < ntiles && p.mask_out != nullptr && lane == 0) p.mask_out[t] = km[0];
+    live = t < ntiles && !(p.skip_empty && km[0] == 0u);
+    if (!live) km[0] = 0u;
+    if (p.reverse_mask && km[0]) reverse_kv(km, p.kv);
+    if (lane == 0) wmask[wave][0] = km[0];
+    __syncthreads();
     uint32_t u = 0u;
 #pragma unroll
-    for (int v = 0; v < kStWaves; ++v) u |= wmask[v][w];
-    U[w] = __builtin_amdgcn_readfirstlane(u);
+    for (int v = 0; v < kStWaves; ++v) u |= wmask[v][0];
+    U[0] = __builtin_amdgcn_readfirstlane(u);
+  } else {
+    // dynamic masks: every OR word of this wave's tile into LDS (lane group
+    // lg loads words lg, lg + 4, ...; one round trip), then windows of MW
+    // words are taken from there in kernel order
+#pragma unroll
+    for (int j = 0; j < kMaxDynWords / 4; ++j) {
+      const int q = lg + 4 * j;
+      uint32_t m = 0u;
+      if (rvalid && q < p.mw) m = p.mask ? p.mask[(int64_t)r * p.mw + q] : 0xffffffffu;
+      m |= __shfl_xor(m, 1);
+      m |= __shfl_xor(m, 2);
+      m |= __shfl_xor(m, 4);
+      m |= __shfl_xor(m, 8);
+      if (lr == 0 && q < p.mw) tmask[wave][q] = m & p.mask_filter & kv_word(p.kv, q);
+    }
+    __syncthreads();
+    uint32_t any = 0u;
+    for (int q = 0; q < p.mw; ++q) any |= tmask[wave][q];
+    any = __builtin_amdgcn_readfirstlane(any);
+    if (t < ntiles && p.mask_out != nullptr)
+      for (int q = lane; q < p.mw; q += 64) p.mask_out[(int64_t)t * p.mw + q] = tmask[wave][q];
+    // a tile past the end or skipped for being empty has no bits: the union
+    // over the block's tiles below needs no live test
+    live = t < ntiles && !(p.skip_empty && any == 0u);
   }
-
   const __amdgpu_buffer_rsrc_t ra = make_rsrc(p.a, p.a_bytes);
   const __amdgpu_buffer_rsrc_t rb = make_rsrc(p.b, p.b_bytes);
   const T* B = reinterpret_cast<const T*>(p.b);
@@ -137,18 +152,20 @@ __global__ __launch_bounds__(64 * kStWaves) void rowgather_stream_kernel(RowGath
 #pragma unroll
   for (int nb = 0; nb < NB; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-  // ---- pipeline over the block's offsets f0 < f1 < f2 < ...
+  // ---- pipeline over the offsets base + f of a window, f0 < f1 < f2 < ...
+  // (window-local f from U; the wave's MFMAs run where its km has f)
+  auto run = [&](int base) {
   const int f0 = next_offset(U, -1);
   if (f0 >= 0) {
     const int f1 = next_offset(U, f0);
     u32x4 wv[WCH];
-    load_w(f0, wv);
-    int i0 = load_idx(f0);
-    int i1 = load_idx(f1);
+    load_w(base + f0, wv);
+    int i0 = load_idx(base + f0);
+    int i1 = load_idx(f1 >= 0 ? base + f1 : -1);
     u32x4 a_cur[KC];
     gather(i0, a_cur);
     store_w(0, wv);
-    load_w(f1, wv);  // weights of f1, stored during f0's turn
+    load_w(f1 >= 0 ? base + f1 : -1, wv);  // weights of f1, stored during f0's turn
     __syncthreads();
     int fc = f0, fn = f1, buf = 0;
     while (fc >= 0) {
@@ -156,7 +173,7 @@ __global__ __launch_bounds__(64 * kStWaves) void rowgather_stream_kernel(RowGath
       // rows of the next offset and pair indices of the one after
       u32x4 a_nxt[KC];
       gather(i1, a_nxt);
-      const int i2 = load_idx(fnn);
+      const int i2 = load_idx(fnn >= 0 ? base + fnn : -1);
       if (mask_bit(km, fc)) {
         const char* wb = wlds + buf * STAGE;
 #pragma unroll
@@ -171,7 +188,7 @@ __global__ __launch_bounds__(64 * kStWaves) void rowgather_stream_kernel(RowGath
         }
       }
       if (fn >= 0) store_w(buf ^ 1, wv);
-      if (fnn >= 0) load_w(fnn, wv);
+      if (fnn >= 0) load_w(base + fnn, wv);
       __syncthreads();  // next stage's weights are in LDS; this stage's reads are done
 #pragma unroll
       for (int kc = 0; kc < KC; ++kc) a_cur[kc] = a_nxt[kc];
@@ -179,6 +196,29 @@ __global__ __launch_bounds__(64 * kStWaves) void rowgather_stream_kernel(RowGath
       fc = fn;
       fn = fnn;
       buf ^= 1;
+    }
+  }
+  };
+  if constexpr (MW == 1) {
+    run(0);
+  } else {
+    const bool rev = p.reverse_mask != 0;
+    auto tile_word = [&](int q) -> uint32_t { return q >= 0 && q < p.mw ? tmask[wave][q] : 0u; };
+    auto union_word = [&](int q) -> uint32_t {
+      uint32_t u = 0u;
+      if (q >= 0 && q < p.mw) {
+#pragma unroll
+        for (int v = 0; v < kStWaves; ++v) u |= tmask[v][q];
+      }
+      return u;
+    };
+    for (int win = 0; 32 * MW * win < p.kv; ++win) {
+#pragma unroll
+      for (int i = 0; i < MW; ++i) {
+        km[i] = live ? __builtin_amdgcn_readfirstlane(kernel_word(tile_word, MW * win + i, p.kv, rev)) : 0u;
+        U[i] = __builtin_amdgcn_readfirstlane(kernel_word(union_word, MW * win + i, p.kv, rev));
+      }
+      run(32 * MW * win);
     }
   }
 
@@ -245,7 +285,7 @@ static hipError_t launch_stream(const RowGatherArgs& a, hipStream_t s, bool* use
   }
   dim3 grid((unsigned)((ntiles + kStWaves - 1) / kStWaves)), block(64 * kStWaves);
   *used = true;
-#define CS_ST(KC_, NB_) hipLaunchKernelGGL((rowgather_stream_kernel<T, OutT, KC_, NB_, MW>), grid, block, 0, s, a, ntiles)
+#define CS_ST(KC_, NB_) CS_LAUNCH((rowgather_stream_kernel<T, OutT, KC_, NB_, MW>), grid, block, 0, s, a, ntiles)
   switch (kc * 16 + nb) {
     case 1 * 16 + 2: CS_ST(1, 2); break;
     case 1 * 16 + 4: CS_ST(1, 4); break;

@@ -514,12 +514,12 @@ static hipError_t launch_ws_kc(const RowGatherArgs& a, int ncol, hipStream_t s) 
   dim3 block(64 * kWsWaves, 1, 1);
   if (ntiles == 0 || grid.y == 0) return hipSuccess;
   switch (ncol / 16) {
-    case 1: hipLaunchKernelGGL((rowgather_ws_kernel<T, OutT, KC, 1>), grid, block, 0, s, a, ntiles); break;
-    case 2: hipLaunchKernelGGL((rowgather_ws_kernel<T, OutT, KC, 2>), grid, block, 0, s, a, ntiles); break;
+    case 1: CS_LAUNCH((rowgather_ws_kernel<T, OutT, KC, 1>), grid, block, 0, s, a, ntiles); break;
+    case 2: CS_LAUNCH((rowgather_ws_kernel<T, OutT, KC, 2>), grid, block, 0, s, a, ntiles); break;
     case 4:
       // four reduction chunks with 64 columns do not fit 128 VGPRs; launch_ws caps them at 32
       if constexpr (KC < 4) {
-        hipLaunchKernelGGL((rowgather_ws_kernel<T, OutT, KC, 4>), grid, block, 0, s, a, ntiles);
+        CS_LAUNCH((rowgather_ws_kernel<T, OutT, KC, 4>), grid, block, 0, s, a, ntiles);
         break;
       }
       return hipErrorInvalidValue;

@@ -1,5 +1,6 @@
 """Dynamic (multi-word) masks on the GPU: 5x5x5 submanifold kernels (kv = 125,
-masks of four 32-bit words per row), through implicit_gemm2 with the
+masks of four 32-bit words per row) and 7x7x7 (kv = 343, eleven words, taken
+by the kernels in windows of four words), through implicit_gemm2 with the
 ``dynamic_mask`` kernels.
 
 * forward, dgrad (forward masks reversed in-kernel) and wgrad (reduced
@@ -12,7 +13,9 @@ masks of four 32-bit words per row), through implicit_gemm2 with the
 * mask_filter acts on bit f % 32 of every word;
 * a 3x3x3 layer through a dynamic-mask desp equals the single-word kernels bit
   for bit (one mask word per row);
-* masks too short for N * words are rejected.
+* masks too short for N * words are rejected; ConvParams.mask_int_count must
+  be its default (1) or the word count; verbose prints the kernel name, its
+  register count and the time.
 """
 import numpy as np
 import pytest
@@ -33,9 +36,9 @@ def _rel(y, y64):
     return ref.relative_error(y, y64)[1]
 
 
-def _case5(n, C, K, seed):
+def _case5(n, C, K, seed, ks=5):
     coords = pc.plane_scene(n, grid=GRID, seed=seed)
-    prob = pc.build_subm(coords, GRID, ksize=(5, 5, 5))
+    prob = pc.build_subm(coords, GRID, ksize=(ks, ks, ks))
     rng = np.random.default_rng(seed + 77)
     X = rng.standard_normal((prob.n, C)).astype(np.float32)
     W = (rng.standard_normal((K, prob.kv, C)) / np.sqrt(prob.kv * C)).astype(np.float32)
@@ -49,7 +52,7 @@ def _fwd(prob, X, W, dt, mask_filter=0xFFFFFFFF):
     assert desp.dynamic_mask
     out = torch.zeros((prob.n, K), dtype=TORCH_DT[dt], device="cuda")
     tiles = (prob.n + 15) // 16
-    mo = torch.zeros((tiles, 4), dtype=torch.int32, device="cuda")
+    mo = torch.zeros((tiles, prob.mask_sorted.shape[1]), dtype=torch.int32, device="cuda")
     p = conv_params(0, desp, to_dev(X, dt), to_dev(W, dt), out, prob.pair_fwd, prob.mask_sorted, prob.argsort,
                     mask_output=mo, mask_filter=mask_filter)
     ConvMainUnitTest.implicit_gemm2(p)
@@ -57,12 +60,15 @@ def _fwd(prob, X, W, dt, mask_filter=0xFFFFFFFF):
     return out.double().cpu().numpy(), mo.cpu().numpy().view(np.uint32)
 
 
-@pytest.mark.parametrize("C,K,dt", [(32, 32, "f16"), (64, 64, "bf16"), (64, 128, "f16"), (16, 32, "f32"),
-                                    (48, 40, "f16"), (32, 160, "bf16")])
-def test_dynamic_forward_dgrad_wgrad(C, K, dt):
-    print(f"[dyn] C={C} K={K} dt={dt}", flush=True)
-    prob, X, W, dO = _case5(2500, C, K, seed=C + K)
-    assert prob.kv == 125 and prob.mask_sorted.shape == (prob.n, 4)
+@pytest.mark.parametrize("C,K,dt,ks", [(32, 32, "f16", 5), (64, 64, "bf16", 5), (64, 128, "f16", 5),
+                                       (16, 32, "f32", 5), (48, 40, "f16", 5), (32, 160, "bf16", 5),
+                                       (32, 32, "f16", 7), (64, 64, "bf16", 7), (16, 32, "f32", 7),
+                                       (48, 40, "f16", 7)])
+def test_dynamic_forward_dgrad_wgrad(C, K, dt, ks):
+    print(f"[dyn] C={C} K={K} dt={dt} ksize={ks}", flush=True)
+    prob, X, W, dO = _case5(2500, C, K, seed=C + K + ks, ks=ks)
+    words = (ks ** 3 + 31) // 32
+    assert prob.kv == ks ** 3 and prob.mask_sorted.shape == (prob.n, words)
     xq, wq, dq = quantize(X, dt), quantize(W, dt), quantize(dO, dt)
     # forward
     got, mo = _fwd(prob, X, W, dt)
@@ -129,3 +135,26 @@ def test_dynamic_masks_validation():
                     prob.argsort)
     with pytest.raises(ValueError, match="dynamic"):
         ConvMainUnitTest.implicit_gemm2(p)
+
+
+def test_dynamic_mask_int_count_and_verbose(capfd):
+    prob, X, W, _ = _case5(800, 16, 16, seed=4, ks=7)
+    desp = desp_for(0, "f16", 16, 16, dynamic=True)
+    out = torch.zeros((prob.n, 16), dtype=torch.float16, device="cuda")
+
+    def params(mic, verbose=False):
+        p = conv_params(0, desp, to_dev(X, "f16"), to_dev(W, "f16"), out, prob.pair_fwd, prob.mask_sorted,
+                        prob.argsort)
+        p.mask_int_count = mic
+        p.verbose = verbose
+        return p
+
+    ConvMainUnitTest.implicit_gemm2(params(1))   # the reference's default
+    ConvMainUnitTest.implicit_gemm2(params(11))  # div_up(343, 32)
+    torch.cuda.synchronize()
+    with pytest.raises(ValueError, match="mask_int_count"):
+        ConvMainUnitTest.implicit_gemm2(params(4))
+    capfd.readouterr()
+    ConvMainUnitTest.implicit_gemm2(params(11, verbose=True))
+    printed = capfd.readouterr().out
+    assert repr(desp) in printed and "kernel num regs:" in printed and "time:" in printed

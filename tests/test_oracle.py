@@ -187,27 +187,30 @@ def test_int8_conv_equals_clipped_integer_bruteforce():
 
 
 # --------------------------------------------------------------- dynamic (multi-word) masks
-def _tiny5(n=160, seed=4, C=4, K=3):
-    grid = (6, 16, 16)
+def _tiny5(n=160, seed=4, C=4, K=3, ks=5):
+    grid = (6, 16, 16) if ks == 5 else (8, 16, 16)
     coords = pc.plane_scene(n, grid=grid, seed=seed)
-    prob = pc.build_subm(coords, grid, ksize=(5, 5, 5))
+    prob = pc.build_subm(coords, grid, ksize=(ks, ks, ks))
     rng = np.random.default_rng(seed + 3)
-    return prob, rng.standard_normal((n, C)), rng.standard_normal((K, 125, C)), rng.standard_normal((n, K))
+    return prob, rng.standard_normal((n, C)), rng.standard_normal((K, ks ** 3, C)), rng.standard_normal((n, K))
 
 
-def test_dynamic_masks_5x5x5_match_bruteforce():
-    """kv = 125: masks are [N, 4] words; forward, dgrad (reversed masks) and
-    wgrad (reduced multi-word masks) equal the coordinate brute force."""
-    prob, X, W, dO = _tiny5()
-    assert prob.kv == 125 and prob.mask_sorted.shape == (prob.n, 4)
+@pytest.mark.parametrize("ks", [5, 7])
+def test_dynamic_masks_match_bruteforce(ks):
+    """kv = 125 / 343: masks are [N, 4] / [N, 11] words; forward, dgrad
+    (reversed masks) and wgrad (reduced multi-word masks) equal the
+    coordinate brute force."""
+    prob, X, W, dO = _tiny5(ks=ks, n=160 if ks == 5 else 120)
+    kv, words = ks ** 3, (ks ** 3 + 31) // 32
+    assert prob.kv == kv and prob.mask_sorted.shape == (prob.n, words)
     assert pc.mac_efficiency(prob.mask_sorted, 16, prob.num_pairs) <= 1.0
     out, tm = ref.conv_forward(X, W, prob.pair_fwd, prob.mask_sorted, prob.argsort, 16)
     assert np.allclose(out, bruteforce.conv_forward_coords(prob.coords, X, W), rtol=1e-12, atol=1e-12)
-    assert tm.shape == ((prob.n + 15) // 16, 4) and np.array_equal(tm, pc.tile_masks(prob.mask_sorted, 16))
+    assert tm.shape == ((prob.n + 15) // 16, words) and np.array_equal(tm, pc.tile_masks(prob.mask_sorted, 16))
     din = ref.conv_dgrad(dO, W, prob.pair_bwd, prob.mask_sorted, prob.argsort, 16, reverse_mask=True)
     assert np.allclose(din, bruteforce.conv_dgrad_coords(prob.coords, dO, W), rtol=1e-12, atol=1e-12)
     dw = ref.conv_wgrad(X, dO, prob.pair_fwd, tm, prob.argsort, 16)
-    assert np.allclose(dw, bruteforce.conv_wgrad_coords(prob.coords, X, dO, 125), rtol=1e-12, atol=1e-12)
+    assert np.allclose(dw, bruteforce.conv_wgrad_coords(prob.coords, X, dO, kv), rtol=1e-12, atol=1e-12)
 
 
 def test_dynamic_mask_filter_is_per_bit_position():
