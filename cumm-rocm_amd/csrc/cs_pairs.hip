@@ -1,21 +1,25 @@
-// Device-side pair tables, per-row masks and mask argsort for 3x3x3 sparse
-// conv layers: what the caller of implicit_gemm2 prepares before each layer.
+// Device-side pair tables, per-row masks and mask argsort for sparse conv
+// layers of any kernel size up to kv = 1024 (3x3x3, 5x5x5, 7x7x7, 2x2x2
+// stride-2 ...): what the caller of implicit_gemm2 prepares before each layer.
 //
 // In the reference that caller is spconv (get_indice_pairs /
 // generate_subm_conv_inds and sort_1d_by_key_split, called from
 // cumm/conv/main_real_spconv.py:91-145), built on tensorview's linear-probing
 // GPU hash table (include/tensorview/hash/linear.cu.h:111-430,
 // hash.cu.h:43-95).  The formats are the ones implicit_gemm2 consumes:
-//   pair[kv][N_out] int32 (-1 = no neighbour), offsets f = kz*9 + ky*3 + kx,
-//   neighbour of a submanifold row = coord + (k - 1), of a strided output
-//   o*2 - 1 + k;  mask[r] bit f <=> pair[f][r] != -1, sorted stably by value
-//   with its argsort.  Coordinates are (b, z, y, x) int32 rows; keys are
-//   ((b*S0 + z)*S1 + y)*S2 + x as uint64.
+//   pair[kv][N_out] int32 (-1 = no neighbour), offsets
+//   f = (kz*K1 + ky)*K2 + kx, neighbour of a submanifold row =
+//   coord + dilation*(k - K/2), of a strided output o*stride - padding + k;
+//   mask[r] bit f % 32 of word f / 32 <=> pair[f][r] != -1 ([N][W] words,
+//   W = div_up(kv, 32)), sorted stably by the words read as one number with
+//   the last word most significant, with its argsort.  Coordinates are
+//   (b, z, y, x) int32 rows; keys are ((b*S0 + z)*S1 + y)*S2 + x as uint64.
 //
 // Kernels: hash insert (open addressing, linear probing, 64-bit CAS), one
 // thread per (offset, row) lookup writing pair_fwd / pair_bwd coalesced along
-// rows, a per-row mask kernel, and rocPRIM radix sorts (stable LSD) for the
-// mask argsort and for the unique output coordinates of strided layers.
+// rows, a per-row mask kernel, and rocPRIM radix sorts (stable LSD, one pass
+// per mask word, least significant word first) for the mask argsort and for
+// the unique output coordinates of strided layers.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 #include <stdint.h>
@@ -28,11 +32,26 @@ namespace cs {
 namespace {
 
 constexpr uint64_t kEmptyKey = ~0ull;
-constexpr int kKv = 27;
+constexpr int kMaxKv = 1024;
 
 struct Shape3 {
   int64_t s0, s1, s2;
 };
+
+// Kernel geometry of one layer (per dimension z, y, x).
+struct Geom {
+  int k0, k1, k2;   // kernel size
+  int st0, st1, st2;  // stride (strided layers)
+  int p0, p1, p2;   // padding (strided layers)
+  int d0, d1, d2;   // dilation (submanifold layers)
+  __host__ __device__ int kv() const { return k0 * k1 * k2; }
+};
+
+__device__ __forceinline__ void offset_of(const Geom& g, int f, int& kz, int& ky, int& kx) {
+  kz = f / (g.k1 * g.k2);
+  ky = (f / g.k2) % g.k1;
+  kx = f % g.k2;
+}
 
 __device__ __forceinline__ uint64_t mix64(uint64_t k) {
   // splitmix64 finaliser
@@ -94,33 +113,38 @@ __device__ __forceinline__ int32_t hash_find(uint64_t key, const uint64_t* __res
   return -1;
 }
 
-// Submanifold: thread (f, i) looks up coord_i + (k_f - 1).  pair_bwd[f] is
-// pair_fwd[26 - f] (the neighbour relation is symmetric).
-__global__ void subm_lookup(const int32_t* __restrict__ coords, int64_t n, Shape3 s, const uint64_t* tkeys,
+// Submanifold: thread (f, i) looks up coord_i + dilation*(k_f - K/2).
+// pair_bwd[f] is pair_fwd[kv - 1 - f] (odd kernel sizes: the neighbour
+// relation is symmetric).
+__global__ void subm_lookup(const int32_t* __restrict__ coords, int64_t n, Shape3 s, Geom g, const uint64_t* tkeys,
                             const int32_t* tvals, uint64_t tmask, int32_t* pair_fwd, int32_t* pair_bwd) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int f = blockIdx.y;
   if (i >= n) return;
   const int32_t* c = coords + 4 * i;
-  const int kz = f / 9, ky = (f / 3) % 3, kx = f % 3;
-  const int64_t z = c[1] + kz - 1, y = c[2] + ky - 1, x = c[3] + kx - 1;
+  int kz, ky, kx;
+  offset_of(g, f, kz, ky, kx);
+  const int64_t z = c[1] + (int64_t)g.d0 * (kz - g.k0 / 2), y = c[2] + (int64_t)g.d1 * (ky - g.k1 / 2),
+                x = c[3] + (int64_t)g.d2 * (kx - g.k2 / 2);
   int32_t j = -1;
   if (z >= 0 && z < s.s0 && y >= 0 && y < s.s1 && x >= 0 && x < s.s2) j = hash_find(key_of(c[0], z, y, x, s), tkeys, tvals, tmask);
   pair_fwd[(int64_t)f * n + i] = j;
-  if (pair_bwd) pair_bwd[(int64_t)(kKv - 1 - f) * n + i] = j;
+  if (pair_bwd) pair_bwd[(int64_t)(g.kv() - 1 - f) * n + i] = j;
 }
 
-// Strided (stride 2, padding 1): thread (f, o) looks up o*2 - 1 + k_f among
-// the inputs; the inverse entry goes to pair_bwd[f][i] (when pair_bwd is set).
+// Strided: thread (f, o) looks up o*stride - padding + k_f among the inputs;
+// the inverse entry goes to pair_bwd[f][i] (when pair_bwd is set).
 __global__ void strided_lookup(const int32_t* __restrict__ out_coords, int64_t n_out, int64_t n_in, Shape3 in_s,
-                               const uint64_t* tkeys, const int32_t* tvals, uint64_t tmask, int32_t* pair_fwd,
-                               int32_t* pair_bwd) {
+                               Geom g, const uint64_t* tkeys, const int32_t* tvals, uint64_t tmask,
+                               int32_t* pair_fwd, int32_t* pair_bwd) {
   const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int f = blockIdx.y;
   if (o >= n_out) return;
   const int32_t* c = out_coords + 4 * o;
-  const int kz = f / 9, ky = (f / 3) % 3, kx = f % 3;
-  const int64_t z = 2 * (int64_t)c[1] - 1 + kz, y = 2 * (int64_t)c[2] - 1 + ky, x = 2 * (int64_t)c[3] - 1 + kx;
+  int kz, ky, kx;
+  offset_of(g, f, kz, ky, kx);
+  const int64_t z = (int64_t)g.st0 * c[1] - g.p0 + kz, y = (int64_t)g.st1 * c[2] - g.p1 + ky,
+                x = (int64_t)g.st2 * c[3] - g.p2 + kx;
   int32_t i = -1;
   if (z >= 0 && z < in_s.s0 && y >= 0 && y < in_s.s1 && x >= 0 && x < in_s.s2)
     i = hash_find(key_of(c[0], z, y, x, in_s), tkeys, tvals, tmask);
@@ -128,35 +152,59 @@ __global__ void strided_lookup(const int32_t* __restrict__ out_coords, int64_t n
   if (pair_bwd != nullptr && i >= 0 && i < n_in) pair_bwd[(int64_t)f * n_in + i] = (int32_t)o;
 }
 
-__global__ void row_masks(const int32_t* __restrict__ pair, int64_t n, uint32_t* mask) {
+// mask[i][w] bit b <=> pair[32w + b][i] >= 0 (W words per row)
+__global__ void row_masks(const int32_t* __restrict__ pair, int64_t n, int kv, uint32_t* mask) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  uint32_t m = 0;
-#pragma unroll
-  for (int f = 0; f < kKv; ++f) m |= (uint32_t)(pair[(int64_t)f * n + i] >= 0) << f;
-  mask[i] = m;
+  const int W = (kv + 31) / 32;
+  for (int w = 0; w < W; ++w) {
+    uint32_t m = 0;
+    const int f1 = min(kv, 32 * w + 32);
+    for (int f = 32 * w; f < f1; ++f) m |= (uint32_t)(pair[(int64_t)f * n + i] >= 0) << (f - 32 * w);
+    mask[i * W + w] = m;
+  }
 }
 
-// Output cell keys of a strided layer: rule 0 ("down") = floor(c / 2) of each
-// input; rule 1 ("reach") = every output whose window o*2 - 1 + k holds the
-// input (27 candidates per input, kEmptyKey where none).
-__global__ void strided_cand_keys(const int32_t* __restrict__ coords, int64_t n, Shape3 out_s, int rule,
+// key[i] = word w of row perm[i] (one LSD pass of the multi-word mask sort)
+__global__ void gather_word(const uint32_t* __restrict__ mask, const int32_t* __restrict__ perm, int64_t n, int W,
+                            int w, uint32_t* key) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    key[i] = mask[(int64_t)perm[i] * W + w];
+}
+
+// sorted[r] = mask[perm[r]] (W words) and argsort[r] = perm[r]
+__global__ void gather_rows(const uint32_t* __restrict__ mask, const int32_t* __restrict__ perm, int64_t n, int W,
+                            uint32_t* sorted, int32_t* argsort) {
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t i = perm[r];
+    for (int w = 0; w < W; ++w) sorted[r * W + w] = mask[(int64_t)i * W + w];
+    argsort[r] = i;
+  }
+}
+
+// Output cell keys of a strided layer: rule 0 ("down") = floor(c / stride) of
+// each input (clamped to the output shape); rule 1 ("reach") = every output
+// whose window o*stride - padding + k holds the input (kv candidates per
+// input, kEmptyKey where none).
+__global__ void strided_cand_keys(const int32_t* __restrict__ coords, int64_t n, Shape3 out_s, Geom g, int rule,
                                   uint64_t* keys) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int32_t* c = coords + 4 * i;
   if (rule == 0) {
-    const int64_t z = std::min<int64_t>(c[1] / 2, out_s.s0 - 1), y = std::min<int64_t>(c[2] / 2, out_s.s1 - 1),
-                  x = std::min<int64_t>(c[3] / 2, out_s.s2 - 1);
+    const int64_t z = std::min<int64_t>(c[1] / g.st0, out_s.s0 - 1), y = std::min<int64_t>(c[2] / g.st1, out_s.s1 - 1),
+                  x = std::min<int64_t>(c[3] / g.st2, out_s.s2 - 1);
     keys[i] = key_of(c[0], z, y, x, out_s);
     return;
   }
-  for (int f = 0; f < kKv; ++f) {
-    const int kz = f / 9, ky = (f / 3) % 3, kx = f % 3;
-    const int64_t nz = c[1] + 1 - kz, ny = c[2] + 1 - ky, nx = c[3] + 1 - kx;
+  const int kv = g.kv();
+  for (int f = 0; f < kv; ++f) {
+    int kz, ky, kx;
+    offset_of(g, f, kz, ky, kx);
+    const int64_t nz = (int64_t)c[1] + g.p0 - kz, ny = (int64_t)c[2] + g.p1 - ky, nx = (int64_t)c[3] + g.p2 - kx;
     uint64_t k = kEmptyKey;
-    if (((nz | ny | nx) & 1) == 0 && nz >= 0 && ny >= 0 && nx >= 0) {
-      const int64_t z = nz / 2, y = ny / 2, x = nx / 2;
+    if (nz >= 0 && ny >= 0 && nx >= 0 && nz % g.st0 == 0 && ny % g.st1 == 0 && nx % g.st2 == 0) {
+      const int64_t z = nz / g.st0, y = ny / g.st1, x = nx / g.st2;
       if (z < out_s.s0 && y < out_s.s1 && x < out_s.s2) k = key_of(c[0], z, y, x, out_s);
     }
     keys[(int64_t)f * n + i] = k;
@@ -206,7 +254,7 @@ int64_t sort_pairs_bytes(int64_t n) {
   size_t t = 0;
   (void)hipcub::DeviceRadixSort::SortPairs(nullptr, t, (const uint32_t*)nullptr, (uint32_t*)nullptr,
                                      (const int32_t*)nullptr, (int32_t*)nullptr, (int)std::max<int64_t>(n, 1),
-                                     0, kKv, (hipStream_t)0);
+                                     0, 32, (hipStream_t)0);
   return (int64_t)t;
 }
 
@@ -237,51 +285,75 @@ int fail(const char* msg, hipError_t e = hipSuccess) {
 
 dim3 grid1(int64_t n) { return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65535))); }
 
-// masks of pair[kv][n] -> stably sorted masks + argsort
-int sort_masks(const int32_t* pair, int64_t n, uint32_t* mask_sorted, int32_t* argsort, Carver& cv,
+// masks of pair[kv][n] -> stably sorted masks ([n][W] words) + argsort.  One
+// word: one key/value radix sort over the kv mask bits.  Several words: LSD
+// passes from word 0 to word W - 1, each a stable sort of the permutation by
+// that word, so the final order is by the words as one number with the last
+// word most significant.
+int sort_masks(const int32_t* pair, int64_t n, int kv, uint32_t* mask_sorted, int32_t* argsort, Carver& cv,
                hipStream_t s) {
-  uint32_t* masks = cv.take<uint32_t>(n);
+  const int W = (kv + 31) / 32;
+  uint32_t* masks = cv.take<uint32_t>(n * W);
   int32_t* iota = cv.take<int32_t>(n);
+  uint32_t* key_in = W > 1 ? cv.take<uint32_t>(n) : nullptr;
+  uint32_t* key_out = W > 1 ? cv.take<uint32_t>(n) : nullptr;
+  int32_t* perm2 = W > 1 ? cv.take<int32_t>(n) : nullptr;
   const int64_t tb = sort_pairs_bytes(n);
   void* tmp = cv.take<char>(tb);
   if (!cv.base || n == 0) return 0;
-  hipLaunchKernelGGL(row_masks, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pair, n, masks);
+  hipLaunchKernelGGL(row_masks, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pair, n, kv, masks);
   PCHK(hipGetLastError());
   hipLaunchKernelGGL(iota_i32, grid1(n), dim3(256), 0, s, iota, n);
   PCHK(hipGetLastError());
   size_t t = (size_t)tb;
-  PCHK(hipcub::DeviceRadixSort::SortPairs(tmp, t, masks, mask_sorted, iota, argsort, (int)n, 0, kKv, s));
+  if (W == 1) {
+    PCHK(hipcub::DeviceRadixSort::SortPairs(tmp, t, masks, mask_sorted, iota, argsort, (int)n, 0, kv, s));
+    return 0;
+  }
+  int32_t* pa = iota;
+  int32_t* pb = perm2;
+  for (int w = 0; w < W; ++w) {
+    hipLaunchKernelGGL(gather_word, grid1(n), dim3(256), 0, s, masks, pa, n, W, w, key_in);
+    PCHK(hipGetLastError());
+    const int end_bit = w == W - 1 ? kv - 32 * w : 32;
+    t = (size_t)tb;
+    PCHK(hipcub::DeviceRadixSort::SortPairs(tmp, t, key_in, key_out, pa, pb, (int)n, 0, end_bit, s));
+    std::swap(pa, pb);
+  }
+  hipLaunchKernelGGL(gather_rows, grid1(n), dim3(256), 0, s, masks, pa, n, W, mask_sorted, argsort);
+  PCHK(hipGetLastError());
   return 0;
 }
 
-int subm_impl(const int32_t* coords, int64_t n, const int32_t* shape, int32_t* pair_fwd, int32_t* pair_bwd,
-              uint32_t* mask_sorted, int32_t* argsort, Carver& cv, hipStream_t s) {
+int subm_impl(const int32_t* coords, int64_t n, const int32_t* shape, const Geom& g, int32_t* pair_fwd,
+              int32_t* pair_bwd, uint32_t* mask_sorted, int32_t* argsort, Carver& cv, hipStream_t s) {
   const Shape3 sh{shape[0], shape[1], shape[2]};
   const uint64_t cap = table_cap(n);
   uint64_t* tkeys = cv.take<uint64_t>((int64_t)cap);
   int32_t* tvals = cv.take<int32_t>((int64_t)cap);
-  if (!cv.base) return sort_masks(nullptr, n, nullptr, nullptr, cv, s);
+  if (!cv.base) return sort_masks(nullptr, n, g.kv(), nullptr, nullptr, cv, s);
   if (n == 0) return 0;
   hipLaunchKernelGGL(fill_u64, grid1((int64_t)cap), dim3(256), 0, s, tkeys, (int64_t)cap, kEmptyKey);
   PCHK(hipGetLastError());
   hipLaunchKernelGGL(hash_insert, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, coords, n, sh, tkeys, tvals,
                      cap - 1);
   PCHK(hipGetLastError());
-  hipLaunchKernelGGL(subm_lookup, dim3((unsigned)((n + 255) / 256), kKv), dim3(256), 0, s, coords, n, sh, tkeys,
-                     tvals, cap - 1, pair_fwd, pair_bwd);
+  hipLaunchKernelGGL(subm_lookup, dim3((unsigned)((n + 255) / 256), (unsigned)g.kv()), dim3(256), 0, s, coords, n,
+                     sh, g, tkeys, tvals, cap - 1, pair_fwd, pair_bwd);
   PCHK(hipGetLastError());
-  return sort_masks(pair_fwd, n, mask_sorted, argsort, cv, s);
+  return sort_masks(pair_fwd, n, g.kv(), mask_sorted, argsort, cv, s);
 }
 
-Shape3 out_shape_of(const int32_t* in_shape) {
-  // (S + 2*padding - k) / stride + 1 with k = 3, padding = 1, stride = 2
-  return Shape3{(in_shape[0] - 1) / 2 + 1, (in_shape[1] - 1) / 2 + 1, (in_shape[2] - 1) / 2 + 1};
+// (S + 2*padding - k) / stride + 1 per dimension
+Shape3 out_shape_of(const int32_t* in_shape, const Geom& g) {
+  return Shape3{(in_shape[0] + 2 * g.p0 - g.k0) / g.st0 + 1, (in_shape[1] + 2 * g.p1 - g.k1) / g.st1 + 1,
+                (in_shape[2] + 2 * g.p2 - g.k2) / g.st2 + 1};
 }
 
-int strided_out_impl(const int32_t* coords, int64_t n, const int32_t* in_shape, int rule, int32_t* out_coords,
-                     int64_t cap, int64_t* n_out, Carver& cv, hipStream_t s) {
-  const Shape3 os = out_shape_of(in_shape);
-  const int64_t nc = rule == 0 ? n : kKv * n;
+int strided_out_impl(const int32_t* coords, int64_t n, const int32_t* in_shape, const Geom& g, int rule,
+                     int32_t* out_coords, int64_t cap, int64_t* n_out, Carver& cv, hipStream_t s) {
+  const Shape3 os = out_shape_of(in_shape, g);
+  const int64_t nc = rule == 0 ? n : (int64_t)g.kv() * n;
   uint64_t* k0 = cv.take<uint64_t>(nc);
   uint64_t* k1 = cv.take<uint64_t>(nc);
   uint64_t* k2 = cv.take<uint64_t>(nc);
@@ -291,7 +363,8 @@ int strided_out_impl(const int32_t* coords, int64_t n, const int32_t* in_shape, 
   if (!cv.base) return 0;
   *n_out = 0;
   if (n == 0) return 0;
-  hipLaunchKernelGGL(strided_cand_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, coords, n, os, rule, k0);
+  hipLaunchKernelGGL(strided_cand_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, coords, n, os, g, rule,
+                     k0);
   PCHK(hipGetLastError());
   size_t t = (size_t)tb;
   PCHK(hipcub::DeviceRadixSort::SortKeys(tmp, t, k0, k1, (int)nc, 0, 64, s));
@@ -315,15 +388,16 @@ int strided_out_impl(const int32_t* coords, int64_t n, const int32_t* in_shape, 
   return 0;
 }
 
-int strided_impl(const int32_t* in_coords, int64_t n_in, const int32_t* in_shape, const int32_t* out_coords,
-                 int64_t n_out, int32_t* pair_fwd, int32_t* pair_bwd, uint32_t* mfs, int32_t* afs, uint32_t* mbs,
-                 int32_t* abs_, Carver& cv, hipStream_t s) {
+int strided_impl(const int32_t* in_coords, int64_t n_in, const int32_t* in_shape, const Geom& g,
+                 const int32_t* out_coords, int64_t n_out, int32_t* pair_fwd, int32_t* pair_bwd, uint32_t* mfs,
+                 int32_t* afs, uint32_t* mbs, int32_t* abs_, Carver& cv, hipStream_t s) {
+  const int kv = g.kv();
   const Shape3 is{in_shape[0], in_shape[1], in_shape[2]};
   const uint64_t cap = table_cap(n_in);
   uint64_t* tkeys = cv.take<uint64_t>((int64_t)cap);
   int32_t* tvals = cv.take<int32_t>((int64_t)cap);
   if (!cv.base) {
-    sort_masks(nullptr, std::max(n_in, n_out), nullptr, nullptr, cv, s);
+    sort_masks(nullptr, std::max(n_in, n_out), kv, nullptr, nullptr, cv, s);
     return 0;
   }
   if (n_in > 0) {
@@ -333,23 +407,62 @@ int strided_impl(const int32_t* in_coords, int64_t n_in, const int32_t* in_shape
                        tkeys, tvals, cap - 1);
     PCHK(hipGetLastError());
     if (pair_bwd != nullptr) {
-      hipLaunchKernelGGL(fill_i32, grid1(kKv * n_in), dim3(256), 0, s, pair_bwd, kKv * n_in, -1);
+      hipLaunchKernelGGL(fill_i32, grid1(kv * n_in), dim3(256), 0, s, pair_bwd, kv * n_in, -1);
       PCHK(hipGetLastError());
     }
   }
   if (n_out > 0) {
-    hipLaunchKernelGGL(strided_lookup, dim3((unsigned)((n_out + 255) / 256), kKv), dim3(256), 0, s, out_coords,
-                       n_out, n_in, is, tkeys, tvals, cap - 1, pair_fwd, pair_bwd);
+    hipLaunchKernelGGL(strided_lookup, dim3((unsigned)((n_out + 255) / 256), (unsigned)kv), dim3(256), 0, s,
+                       out_coords, n_out, n_in, is, g, tkeys, tvals, cap - 1, pair_fwd, pair_bwd);
     PCHK(hipGetLastError());
   }
   const int64_t mark = cv.off;
-  int r = sort_masks(pair_fwd, n_out, mfs, afs, cv, s);
+  int r = sort_masks(pair_fwd, n_out, kv, mfs, afs, cv, s);
   if (r || pair_bwd == nullptr) return r;  // forward-only tables (inference)
   cv.off = mark;  // the backward sort reuses the scratch; both are ordered on one stream
-  return sort_masks(pair_bwd, n_in, mbs, abs_, cv, s);
+  return sort_masks(pair_bwd, n_in, kv, mbs, abs_, cv, s);
 }
 
 bool bad_shape(const int32_t* s) { return s == nullptr || s[0] <= 0 || s[1] <= 0 || s[2] <= 0; }
+
+Geom geom_of(const cs_conv_geom* g) {
+  Geom r;
+  r.k0 = g->ksize[0], r.k1 = g->ksize[1], r.k2 = g->ksize[2];
+  r.st0 = g->stride[0], r.st1 = g->stride[1], r.st2 = g->stride[2];
+  r.p0 = g->padding[0], r.p1 = g->padding[1], r.p2 = g->padding[2];
+  r.d0 = g->dilation[0], r.d1 = g->dilation[1], r.d2 = g->dilation[2];
+  return r;
+}
+
+// 3x3x3; strided layers stride 2, padding 1
+cs_conv_geom default_geom() {
+  cs_conv_geom g;
+  for (int i = 0; i < 3; ++i) {
+    g.ksize[i] = 3;
+    g.stride[i] = 2;
+    g.padding[i] = 1;
+    g.dilation[i] = 1;
+  }
+  return g;
+}
+
+// nullptr when the geometry is usable, else the reason
+const char* bad_geom(const cs_conv_geom* g, bool subm) {
+  if (g == nullptr) return "null geometry";
+  int64_t kv = 1;
+  for (int i = 0; i < 3; ++i) {
+    if (g->ksize[i] < 1) return "kernel sizes must be positive";
+    kv *= g->ksize[i];
+    if (subm) {
+      if (g->ksize[i] % 2 == 0) return "submanifold kernel sizes must be odd";
+      if (g->dilation[i] < 1) return "dilation must be positive";
+    } else {
+      if (g->stride[i] < 1 || g->padding[i] < 0) return "stride must be positive and padding non-negative";
+    }
+  }
+  if (kv > kMaxKv) return "kernel volume above 1024";
+  return nullptr;
+}
 
 }  // namespace
 }  // namespace cs
@@ -358,66 +471,100 @@ extern "C" {
 
 const char* cs_pairs_last_error(void) { return cs::g_err; }
 
-int cs_pairs_subm_workspace(int64_t n, int64_t* nbytes) {
+int cs_pairs_subm_geom_workspace(int64_t n, const cs_conv_geom* geom, int64_t* nbytes) {
   if (nbytes == nullptr || n < 0) return cs::fail("bad argument");
+  if (const char* why = cs::bad_geom(geom, true)) return cs::fail(why);
   cs::Carver cv{nullptr};
   const int32_t shape[3] = {1, 1, 1};
-  cs::subm_impl(nullptr, n, shape, nullptr, nullptr, nullptr, nullptr, cv, 0);
+  cs::subm_impl(nullptr, n, shape, cs::geom_of(geom), nullptr, nullptr, nullptr, nullptr, cv, 0);
   *nbytes = cv.off;
   return 0;
+}
+
+int cs_pairs_subm_geom(const int32_t* coords, int64_t n, const int32_t* spatial_shape, const cs_conv_geom* geom,
+                       int32_t* pair_fwd, int32_t* pair_bwd, uint32_t* mask_sorted, int32_t* argsort, void* workspace,
+                       int64_t ws_bytes, uint64_t stream) {
+  int64_t need = 0;
+  if (cs_pairs_subm_geom_workspace(n, geom, &need)) return 1;
+  if (n > 0x7fffffff || cs::bad_shape(spatial_shape)) return cs::fail("bad size or spatial shape");
+  if (ws_bytes < need || (n > 0 && (!coords || !pair_fwd || !mask_sorted || !argsort || !workspace)))
+    return cs::fail("null buffer or workspace too small");
+  cs::Carver cv{static_cast<char*>(workspace)};
+  return cs::subm_impl(coords, n, spatial_shape, cs::geom_of(geom), pair_fwd, pair_bwd, mask_sorted, argsort, cv,
+                       reinterpret_cast<hipStream_t>(stream));
+}
+
+int cs_pairs_subm_workspace(int64_t n, int64_t* nbytes) {
+  const cs_conv_geom g = cs::default_geom();
+  return cs_pairs_subm_geom_workspace(n, &g, nbytes);
 }
 
 int cs_pairs_subm(const int32_t* coords, int64_t n, const int32_t* spatial_shape, int32_t* pair_fwd,
                   int32_t* pair_bwd, uint32_t* mask_sorted, int32_t* argsort, void* workspace, int64_t ws_bytes,
                   uint64_t stream) {
-  int64_t need = 0;
-  cs_pairs_subm_workspace(n, &need);
-  if (n < 0 || n > 0x7fffffff || cs::bad_shape(spatial_shape)) return cs::fail("bad size or spatial shape");
-  if (ws_bytes < need || (n > 0 && (!coords || !pair_fwd || !mask_sorted || !argsort || !workspace)))
-    return cs::fail("null buffer or workspace too small");
-  cs::Carver cv{static_cast<char*>(workspace)};
-  return cs::subm_impl(coords, n, spatial_shape, pair_fwd, pair_bwd, mask_sorted, argsort, cv,
-                       reinterpret_cast<hipStream_t>(stream));
+  const cs_conv_geom g = cs::default_geom();
+  return cs_pairs_subm_geom(coords, n, spatial_shape, &g, pair_fwd, pair_bwd, mask_sorted, argsort, workspace,
+                            ws_bytes, stream);
 }
 
-int cs_pairs_strided_out_workspace(int64_t n, int32_t rule, int64_t* nbytes) {
+int cs_pairs_strided_out_geom_workspace(int64_t n, const cs_conv_geom* geom, int32_t rule, int64_t* nbytes) {
   if (nbytes == nullptr || n < 0 || (rule != 0 && rule != 1)) return cs::fail("bad argument");
+  if (const char* why = cs::bad_geom(geom, false)) return cs::fail(why);
   cs::Carver cv{nullptr};
   const int32_t shape[3] = {1, 1, 1};
   int64_t dummy = 0;
-  cs::strided_out_impl(nullptr, n, shape, rule, nullptr, 0, &dummy, cv, 0);
+  cs::strided_out_impl(nullptr, n, shape, cs::geom_of(geom), rule, nullptr, 0, &dummy, cv, 0);
   *nbytes = cv.off;
   return 0;
+}
+
+int cs_pairs_strided_out_geom(const int32_t* coords, int64_t n, const int32_t* in_shape, const cs_conv_geom* geom,
+                              int32_t rule, int32_t* out_coords, int64_t cap, int64_t* n_out, void* workspace,
+                              int64_t ws_bytes, uint64_t stream) {
+  int64_t need = 0;
+  if (cs_pairs_strided_out_geom_workspace(n, geom, rule, &need)) return 1;
+  const cs::Geom g = cs::geom_of(geom);
+  if (n > 0x7fffffff / g.kv() || cs::bad_shape(in_shape)) return cs::fail("bad size or spatial shape");
+  const cs::Shape3 os = cs::out_shape_of(in_shape, g);
+  if (os.s0 < 1 || os.s1 < 1 || os.s2 < 1) return cs::fail("kernel larger than the padded input");
+  if (ws_bytes < need || n_out == nullptr || (n > 0 && (!coords || !out_coords || !workspace)))
+    return cs::fail("null buffer or workspace too small");
+  cs::Carver cv{static_cast<char*>(workspace)};
+  return cs::strided_out_impl(coords, n, in_shape, g, rule, out_coords, cap, n_out, cv,
+                              reinterpret_cast<hipStream_t>(stream));
+}
+
+int cs_pairs_strided_out_workspace(int64_t n, int32_t rule, int64_t* nbytes) {
+  const cs_conv_geom g = cs::default_geom();
+  return cs_pairs_strided_out_geom_workspace(n, &g, rule, nbytes);
 }
 
 int cs_pairs_strided_out(const int32_t* coords, int64_t n, const int32_t* in_shape, int32_t rule, int32_t* out_coords,
                          int64_t cap, int64_t* n_out, void* workspace, int64_t ws_bytes, uint64_t stream) {
-  int64_t need = 0;
-  if (cs_pairs_strided_out_workspace(n, rule, &need)) return 1;
-  if (n > 0x7fffffff / 27 || cs::bad_shape(in_shape)) return cs::fail("bad size or spatial shape");
-  if (ws_bytes < need || n_out == nullptr || (n > 0 && (!coords || !out_coords || !workspace)))
-    return cs::fail("null buffer or workspace too small");
-  cs::Carver cv{static_cast<char*>(workspace)};
-  return cs::strided_out_impl(coords, n, in_shape, rule, out_coords, cap, n_out, cv,
-                              reinterpret_cast<hipStream_t>(stream));
+  const cs_conv_geom g = cs::default_geom();
+  return cs_pairs_strided_out_geom(coords, n, in_shape, &g, rule, out_coords, cap, n_out, workspace, ws_bytes,
+                                   stream);
 }
 
-int cs_pairs_strided_workspace(int64_t n_in, int64_t n_out, int64_t* nbytes) {
+int cs_pairs_strided_geom_workspace(int64_t n_in, int64_t n_out, const cs_conv_geom* geom, int64_t* nbytes) {
   if (nbytes == nullptr || n_in < 0 || n_out < 0) return cs::fail("bad argument");
+  if (const char* why = cs::bad_geom(geom, false)) return cs::fail(why);
   cs::Carver cv{nullptr};
   const int32_t shape[3] = {1, 1, 1};
-  cs::strided_impl(nullptr, n_in, shape, nullptr, n_out, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, cv, 0);
+  cs::strided_impl(nullptr, n_in, shape, cs::geom_of(geom), nullptr, n_out, nullptr, nullptr, nullptr, nullptr,
+                   nullptr, nullptr, cv, 0);
   *nbytes = cv.off;
   return 0;
 }
 
-int cs_pairs_strided(const int32_t* in_coords, int64_t n_in, const int32_t* in_shape, const int32_t* out_coords,
-                     int64_t n_out, int32_t* pair_fwd, int32_t* pair_bwd, uint32_t* mask_fwd_sorted,
-                     int32_t* argsort_fwd, uint32_t* mask_bwd_sorted, int32_t* argsort_bwd, void* workspace,
-                     int64_t ws_bytes, uint64_t stream) {
+int cs_pairs_strided_geom(const int32_t* in_coords, int64_t n_in, const int32_t* in_shape, const cs_conv_geom* geom,
+                          const int32_t* out_coords, int64_t n_out, int32_t* pair_fwd, int32_t* pair_bwd,
+                          uint32_t* mask_fwd_sorted, int32_t* argsort_fwd, uint32_t* mask_bwd_sorted,
+                          int32_t* argsort_bwd, void* workspace, int64_t ws_bytes, uint64_t stream) {
   int64_t need = 0;
-  cs_pairs_strided_workspace(n_in, n_out, &need);
-  if (n_in > 0x7fffffff / 27 || n_out > 0x7fffffff / 27 || cs::bad_shape(in_shape))
+  if (cs_pairs_strided_geom_workspace(n_in, n_out, geom, &need)) return 1;
+  const cs::Geom g = cs::geom_of(geom);
+  if (n_in > 0x7fffffff / g.kv() || n_out > 0x7fffffff / g.kv() || cs::bad_shape(in_shape))
     return cs::fail("bad size or spatial shape");
   // the backward side (pair_bwd, mask_bwd_sorted, argsort_bwd) is all set or all null
   const bool bwd = pair_bwd != nullptr;
@@ -427,8 +574,22 @@ int cs_pairs_strided(const int32_t* in_coords, int64_t n_in, const int32_t* in_s
       (n_in > 0 && !in_coords))
     return cs::fail("null buffer or workspace too small");
   cs::Carver cv{static_cast<char*>(workspace)};
-  return cs::strided_impl(in_coords, n_in, in_shape, out_coords, n_out, pair_fwd, pair_bwd, mask_fwd_sorted,
+  return cs::strided_impl(in_coords, n_in, in_shape, g, out_coords, n_out, pair_fwd, pair_bwd, mask_fwd_sorted,
                           argsort_fwd, mask_bwd_sorted, argsort_bwd, cv, reinterpret_cast<hipStream_t>(stream));
+}
+
+int cs_pairs_strided_workspace(int64_t n_in, int64_t n_out, int64_t* nbytes) {
+  const cs_conv_geom g = cs::default_geom();
+  return cs_pairs_strided_geom_workspace(n_in, n_out, &g, nbytes);
+}
+
+int cs_pairs_strided(const int32_t* in_coords, int64_t n_in, const int32_t* in_shape, const int32_t* out_coords,
+                     int64_t n_out, int32_t* pair_fwd, int32_t* pair_bwd, uint32_t* mask_fwd_sorted,
+                     int32_t* argsort_fwd, uint32_t* mask_bwd_sorted, int32_t* argsort_bwd, void* workspace,
+                     int64_t ws_bytes, uint64_t stream) {
+  const cs_conv_geom g = cs::default_geom();
+  return cs_pairs_strided_geom(in_coords, n_in, in_shape, &g, out_coords, n_out, pair_fwd, pair_bwd, mask_fwd_sorted,
+                               argsort_fwd, mask_bwd_sorted, argsort_bwd, workspace, ws_bytes, stream);
 }
 
 }  // extern "C"

@@ -29,6 +29,8 @@ SYMBOLS = [
     "cs_stream_synchronize", "cs_last_error",
     "cs_pairs_subm_workspace", "cs_pairs_subm", "cs_pairs_strided_out_workspace", "cs_pairs_strided_out",
     "cs_pairs_strided_workspace", "cs_pairs_strided", "cs_pairs_last_error",
+    "cs_pairs_subm_geom_workspace", "cs_pairs_subm_geom", "cs_pairs_strided_out_geom_workspace",
+    "cs_pairs_strided_out_geom", "cs_pairs_strided_geom_workspace", "cs_pairs_strided_geom",
     "cs_get_all_gather_params", "cs_get_all_scatter_params", "cs_gather_supported", "cs_scatter_supported",
     "cs_gather", "cs_scatter",
 ]
@@ -66,6 +68,12 @@ class CsConvAlgoDesp(C.Structure):
         ("mask_sparse", C.c_int32), ("increment_k_first", C.c_int32),
         ("dynamic_mask", C.c_int32), ("is_int8_inference", C.c_int32),
     ]
+
+
+class CsConvGeom(C.Structure):
+    """cs_conv_geom: kernel size, stride, padding, dilation per (z, y, x)."""
+    _fields_ = [("ksize", C.c_int32 * 3), ("stride", C.c_int32 * 3), ("padding", C.c_int32 * 3),
+                ("dilation", C.c_int32 * 3)]
 
 
 class CsGemmParams(C.Structure):
@@ -136,8 +144,17 @@ def lib() -> C.CDLL:
     L.cs_pairs_strided_out.argtypes = [P, I64, P, I32, P, I64, C.POINTER(I64), P, I64, U64]
     L.cs_pairs_strided_workspace.argtypes = [I64, I64, C.POINTER(I64)]
     L.cs_pairs_strided.argtypes = [P, I64, P, P, I64, P, P, P, P, P, P, P, I64, U64]
+    G = C.POINTER(CsConvGeom)
+    L.cs_pairs_subm_geom_workspace.argtypes = [I64, G, C.POINTER(I64)]
+    L.cs_pairs_subm_geom.argtypes = [P, I64, P, G, P, P, P, P, P, I64, U64]
+    L.cs_pairs_strided_out_geom_workspace.argtypes = [I64, G, I32, C.POINTER(I64)]
+    L.cs_pairs_strided_out_geom.argtypes = [P, I64, P, G, I32, P, I64, C.POINTER(I64), P, I64, U64]
+    L.cs_pairs_strided_geom_workspace.argtypes = [I64, I64, G, C.POINTER(I64)]
+    L.cs_pairs_strided_geom.argtypes = [P, I64, P, G, P, I64, P, P, P, P, P, P, P, I64, U64]
     for name in ("cs_pairs_subm_workspace", "cs_pairs_subm", "cs_pairs_strided_out_workspace",
-                 "cs_pairs_strided_out", "cs_pairs_strided_workspace", "cs_pairs_strided"):
+                 "cs_pairs_strided_out", "cs_pairs_strided_workspace", "cs_pairs_strided",
+                 "cs_pairs_subm_geom_workspace", "cs_pairs_subm_geom", "cs_pairs_strided_out_geom_workspace",
+                 "cs_pairs_strided_out_geom", "cs_pairs_strided_geom_workspace", "cs_pairs_strided_geom"):
         getattr(L, name).restype = C.c_int
     L.cs_pairs_last_error.restype = C.c_char_p
     L.cs_get_all_gather_params.argtypes = [P, C.c_int]

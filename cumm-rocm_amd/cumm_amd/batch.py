@@ -24,6 +24,7 @@ from typing import List, Optional, Sequence, Tuple
 import numpy as np
 import torch
 
+from . import pairs
 from . import pointcloud as pc
 from . import tensorview as tv
 from .conv.main import ConvMainUnitTest, select_conv_algo
@@ -74,21 +75,24 @@ class LayerTables:
         return int(np.count_nonzero(self.pair_fwd >= 0))
 
 
-def build_tables(kind: str, coords: np.ndarray, shape: Sequence[int], rule: str = "down") -> LayerTables:
-    """Pair tables of a ``subm`` or stride-2 ``strided`` 3x3x3 layer; strided
-    outputs follow ``rule`` (``down``: unique(floor(c / 2)), the downsampling
-    of the batched and serving stacks; ``reach``: every reached position)."""
+def build_tables(kind, coords: np.ndarray, shape: Sequence[int], rule: str = "down") -> LayerTables:
+    """Host pair tables of one layer; ``kind`` goes through
+    ``pairs.layer_spec`` ("subm" / "strided" are 3x3x3, strided stride 2,
+    padding 1).  Strided outputs follow ``rule`` (``down``: unique(floor(c /
+    stride)), the downsampling of the batched and serving stacks; ``reach``:
+    every reached position)."""
+    spec = pairs.layer_spec(kind)
     shape = tuple(int(v) for v in shape)
-    if kind == "subm":
-        prob = pc.build_subm(coords, shape)
+    if spec.kind == "subm":
+        prob = pc.build_subm(coords, shape, spec.ksize, dilation=np.asarray(spec.dilation))
         return LayerTables("subm", coords, coords, shape, shape, prob.pair_fwd, prob.pair_bwd,
                            prob.mask_sorted, prob.argsort, None, None)
-    sp = pc.build_strided(coords, shape, rule=rule)
+    sp = pc.build_strided(coords, shape, spec.ksize, np.asarray(spec.stride), np.asarray(spec.padding), rule=rule)
     return LayerTables("strided", coords, sp.out_coords, shape, sp.out_shape, sp.pair_fwd, sp.pair_bwd,
                        sp.mask_fwd_sorted, sp.argsort_fwd, sp.mask_bwd_sorted, sp.argsort_bwd)
 
 
-def build_stack_tables(kinds: Sequence[str], coords: np.ndarray, shape: Sequence[int]) -> List[LayerTables]:
+def build_stack_tables(kinds: Sequence, coords: np.ndarray, shape: Sequence[int]) -> List[LayerTables]:
     out = []
     for k in kinds:
         t = build_tables(k, coords, shape)
@@ -138,9 +142,12 @@ class DeviceLayer:
             self.din = torch.zeros((t.n_in, C), dtype=td, device=dev)
         else:
             self.dw = self.dout = self.din = None
+        # kernel volumes above 32 take the dynamic (multi-word) mask kernels
+        self.dynamic = kv > 32
+        self.mw = (kv + 31) // 32
         # the forward writes every tile's mask; zero-filled only where wgrad reads it
         alloc = torch.zeros if train else torch.empty
-        self.mask_out = alloc(((t.n_out + 15) // 16 + 1,), dtype=torch.int32, device=dev)
+        self.mask_out = alloc((((t.n_out + 15) // 16 + 1) * self.mw,), dtype=torch.int32, device=dev)
         self.stream = stream
         self.x = None
         self.params = {}
@@ -156,7 +163,7 @@ class DeviceLayer:
 
         def mk(op: int) -> ConvParams:
             p = ConvParams(3, ConvOpType(op))
-            p.conv_algo_desp = select_conv_algo(op, tvdt, tvdt, tvdt, C, K)
+            p.conv_algo_desp = select_conv_algo(op, tvdt, tvdt, tvdt, C, K, self.dynamic)
             p.mask_argsort = tv.from_torch(self.af)
             p.stream = self.stream
             if op == 0:

@@ -287,8 +287,8 @@ class SubmProblem:
 
 
 def build_subm(coords: np.ndarray, spatial_shape: Sequence[int],
-               ksize: Sequence[int] = (3, 3, 3)) -> SubmProblem:
-    pf = subm_pairs(coords, spatial_shape, ksize)
+               ksize: Sequence[int] = (3, 3, 3), dilation=1) -> SubmProblem:
+    pf = subm_pairs(coords, spatial_shape, ksize, dilation)
     kv = pf.shape[0]
     # subm symmetry: coord_j = coord_i + d_f  <=>  coord_i = coord_j + d_{kv-1-f}
     pb = pf[::-1].copy()

@@ -69,7 +69,7 @@ class _Batch:
 
 
 class SparseConvServer:
-    def __init__(self, kinds: Sequence[str], channels: Sequence[int], weights: Sequence[np.ndarray], dt: str = "bf16",
+    def __init__(self, kinds: Sequence, channels: Sequence[int], weights: Sequence[np.ndarray], dt: str = "bf16",
                  hbm_budget: int = 8 << 30, max_inflight: int = 64, max_batch: int = 16, n_streams: int = 1,
                  grid=GRID, keep_outputs: bool = True):
         assert len(kinds) + 1 == len(channels) == len(weights) + 1
@@ -124,7 +124,9 @@ class SparseConvServer:
         per_row = 16 + 4 * self.channels[0] + es * self.channels[0]
         for c in self.channels[1:]:
             per_row += es * c + 16
-        per_row += len(self.kinds) * (27 * 4 * 2 + 8 * 4 + 64)
+        for k in self.kinds:
+            kv = pairs.layer_spec(k).kv
+            per_row += kv * 4 * 2 + 8 * 4 * ((kv + 31) // 32) + 64
         return int(per_row * n * 1.25) + (4 << 20)
 
     def now(self) -> float:

@@ -161,28 +161,54 @@ int cs_stream_synchronize(uint64_t stream);
 /* Message of the last failing call on this thread ("" if none). */
 const char* cs_last_error(void);
 
-/* ---- Pair tables, masks and argsort on the device (3x3x3 kernels).
+/* ---- Pair tables, masks and argsort on the device.
  * These replace what the caller of implicit_gemm2 builds before each layer:
  * spconv's get_indice_pairs / generate_subm_conv_inds + sort_1d_by_key_split
  * (called at cumm/conv/main_real_spconv.py:91-145), over tensorview's GPU
  * linear-probing hash table (include/tensorview/hash/linear.cu.h:111-430).
  * Coordinates are int32 rows (b, z, y, x); spatial shapes are (z, y, x).
- * Outputs: pair tables [27][N] int32 (-1 = none), masks uint32 sorted stably
- * with their argsort.  All buffers are device memory; work is queued on
- * `stream`.  Return 0 / 1 (bad argument) / 2 (device error), message in
- * cs_pairs_last_error(). */
+ * Outputs: pair tables [kv][N] int32 (-1 = none) with offsets
+ * f = (kz*K1 + ky)*K2 + kx, masks [N][W] uint32 (W = div_up(kv, 32), offset f
+ * in bit f % 32 of word f / 32) sorted stably by the words read as one number
+ * (last word most significant), with their argsort.  All buffers are device
+ * memory; work is queued on `stream`.  Return 0 / 1 (bad argument) / 2
+ * (device error), message in cs_pairs_last_error().
+ * The *_geom entry points take the layer's kernel geometry (kv <= 1024):
+ * submanifold neighbours coord + dilation*(k - K/2) (odd K), strided outputs
+ * o*stride - padding + k over an output shape (S + 2*padding - K)/stride + 1.
+ * The entry points without _geom are 3x3x3 (strided: stride 2, padding 1). */
+typedef struct cs_conv_geom {
+  int32_t ksize[3];
+  int32_t stride[3];
+  int32_t padding[3];
+  int32_t dilation[3];
+} cs_conv_geom;
+int cs_pairs_subm_geom_workspace(int64_t n, const cs_conv_geom* geom, int64_t* nbytes);
+int cs_pairs_subm_geom(const int32_t* coords, int64_t n, const int32_t* spatial_shape, const cs_conv_geom* geom,
+                       int32_t* pair_fwd, int32_t* pair_bwd, uint32_t* mask_sorted, int32_t* argsort,
+                       void* workspace, int64_t ws_bytes, uint64_t stream);
+int cs_pairs_strided_out_geom_workspace(int64_t n, const cs_conv_geom* geom, int32_t rule, int64_t* nbytes);
+int cs_pairs_strided_out_geom(const int32_t* coords, int64_t n, const int32_t* in_shape, const cs_conv_geom* geom,
+                              int32_t rule, int32_t* out_coords, int64_t cap, int64_t* n_out, void* workspace,
+                              int64_t ws_bytes, uint64_t stream);
+int cs_pairs_strided_geom_workspace(int64_t n_in, int64_t n_out, const cs_conv_geom* geom, int64_t* nbytes);
+int cs_pairs_strided_geom(const int32_t* in_coords, int64_t n_in, const int32_t* in_shape, const cs_conv_geom* geom,
+                          const int32_t* out_coords, int64_t n_out, int32_t* pair_fwd, int32_t* pair_bwd,
+                          uint32_t* mask_fwd_sorted, int32_t* argsort_fwd, uint32_t* mask_bwd_sorted,
+                          int32_t* argsort_bwd, void* workspace, int64_t ws_bytes, uint64_t stream);
 int cs_pairs_subm_workspace(int64_t n, int64_t* nbytes);
 int cs_pairs_subm(const int32_t* coords, int64_t n, const int32_t* spatial_shape, int32_t* pair_fwd,
                   int32_t* pair_bwd, uint32_t* mask_sorted, int32_t* argsort, void* workspace,
                   int64_t ws_bytes, uint64_t stream);
-/* Output coordinates of a stride-2, padding-1 layer, sorted by key: rule 0 =
- * unique(floor(c / 2)), rule 1 = every output whose window reaches an input.
- * Synchronises `stream` to return the count in *n_out (<= cap rows written). */
+/* Output coordinates of a strided layer, sorted by key: rule 0 =
+ * unique(floor(c / stride)), rule 1 = every output whose window reaches an
+ * input.  Synchronises `stream` to return the count in *n_out (<= cap rows
+ * written). */
 int cs_pairs_strided_out_workspace(int64_t n, int32_t rule, int64_t* nbytes);
 int cs_pairs_strided_out(const int32_t* coords, int64_t n, const int32_t* in_shape, int32_t rule,
                          int32_t* out_coords, int64_t cap, int64_t* n_out, void* workspace, int64_t ws_bytes,
                          uint64_t stream);
-/* Strided pair tables: pair_fwd [27][n_out] (input rows), pair_bwd [27][n_in]
+/* Strided pair tables: pair_fwd [kv][n_out] (input rows), pair_bwd [kv][n_in]
  * (output rows), forward masks over the outputs and backward masks over the
  * inputs, each sorted with its argsort.  Forward-only callers (inference)
  * pass pair_bwd, mask_bwd_sorted and argsort_bwd all NULL; cs_pairs_subm

@@ -139,3 +139,69 @@ def test_tables_on_a_side_stream(kind):
         assert np.array_equal(_np(dev.pair_bwd), host.pair_bwd)
         assert np.array_equal(_np(dev.mask_fwd).view(np.uint32), host.mask_fwd_sorted)
         assert np.array_equal(_np(dev.argsort_bwd), host.argsort_bwd)
+
+
+@pytest.mark.parametrize("ksize,dilation", [((5, 5, 5), 1), ((7, 7, 7), 1), ((3, 3, 3), 2), ((5, 3, 3), (1, 2, 1))])
+def test_subm_geometry_tables_match_host(ksize, dilation):
+    """Other kernel sizes and dilations; kv > 32 gives [N, W] masks sorted by
+    the words as one number (last word most significant)."""
+    print(f"[pairs] subm ksize={ksize} dilation={dilation}", flush=True)
+    grid = (16, 96, 96)
+    c = _coords([3000, 800], grid, 13)
+    host = pc.build_subm(c, grid, ksize, dilation=np.asarray(dilation))
+    dev = pairs.subm_tables(torch.from_numpy(c).cuda(), grid, ksize=ksize, dilation=dilation)
+    torch.cuda.synchronize()
+    assert dev.pair_fwd.shape[0] == host.kv
+    assert np.array_equal(_np(dev.pair_fwd), host.pair_fwd)
+    assert np.array_equal(_np(dev.pair_bwd), host.pair_bwd)
+    assert np.array_equal(_np(dev.mask_fwd).view(np.uint32), host.mask_sorted)
+    assert np.array_equal(_np(dev.argsort_fwd), host.argsort)
+
+
+@pytest.mark.parametrize("rule", ["down", "reach"])
+@pytest.mark.parametrize("ksize,stride,padding", [((2, 2, 2), 2, 0), ((3, 3, 3), 1, 1), ((5, 5, 5), 2, 2),
+                                                  ((3, 3, 3), (2, 2, 1), (1, 1, 0))])
+def test_strided_geometry_tables_match_host(ksize, stride, padding, rule):
+    print(f"[pairs] strided ksize={ksize} stride={stride} padding={padding} {rule}", flush=True)
+    grid = (16, 96, 96)
+    c = _coords([3000, 800], grid, 17)
+    st, pd = np.broadcast_to(np.asarray(stride), 3), np.broadcast_to(np.asarray(padding), 3)
+    host = pc.build_strided(c, grid, ksize, st, pd, rule=rule)
+    dev = pairs.strided_tables(torch.from_numpy(c).cuda(), grid, rule=rule, ksize=ksize, stride=stride,
+                               padding=padding)
+    torch.cuda.synchronize()
+    assert dev.out_shape == tuple(host.out_shape)
+    assert np.array_equal(_np(dev.out_coords), host.out_coords)
+    assert np.array_equal(_np(dev.pair_fwd), host.pair_fwd)
+    assert np.array_equal(_np(dev.pair_bwd), host.pair_bwd)
+    assert np.array_equal(_np(dev.mask_fwd).view(np.uint32), host.mask_fwd_sorted)
+    assert np.array_equal(_np(dev.argsort_fwd), host.argsort_fwd)
+    assert np.array_equal(_np(dev.mask_bwd).view(np.uint32), host.mask_bwd_sorted)
+    assert np.array_equal(_np(dev.argsort_bwd), host.argsort_bwd)
+
+
+def test_device_tables_drive_a_dynamic_stack():
+    """5x5x5 submanifold layers (dynamic masks) and a 2x2x2 stride-2 layer,
+    device-resident: the stack gives the same bits over device and host tables."""
+    grid = (16, 96, 96)
+    kinds = (("subm", 5), ("strided", 2, 2, 0), ("subm", 5))
+    c = _coords([2500, 700], grid, 19)
+    ch = (16, 32, 32, 64)
+    kvs = (125, 8, 125)
+    rng = np.random.default_rng(4)
+    Ws = [rng.normal(0, 0.05, (ch[i + 1], kvs[i], ch[i])).astype(np.float32) for i in range(3)]
+    host = B.build_stack_tables(kinds, c, grid)
+    dev = pairs.stack_tables(kinds, torch.from_numpy(c).cuda(), grid)
+    assert [t.pair_fwd.shape[0] for t in host] == list(kvs)
+    X = rng.standard_normal((c.shape[0], ch[0])).astype(np.float32)
+    dO = rng.standard_normal((host[-1].n_out, ch[-1])).astype(np.float32)
+    outs = []
+    for tabs in (host, dev):
+        st = B.ConvStack(tabs, ch, "f16", Ws, torch.cuda.current_stream().cuda_stream)
+        st.bind(torch.from_numpy(X).cuda().half(), torch.from_numpy(dO).cuda().half())
+        st.step()
+        torch.cuda.synchronize()
+        outs.append([st.layers[-1].out.clone(), st.layers[2].din.clone(), st.layers[1].din.clone(),
+                     st.layers[0].dw.clone(), st.layers[2].dw.clone()])
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)
