@@ -861,6 +861,13 @@ GemmPlan plan_gemm(const cs_gemm_params& p) {
   // GemmParams::check_valid  (params.h:351-364) + nvrtc_gemm_template (nvrtc_code.py:34-145)
   const cs_gemm_algo_desp& desp = p.algo_desp;
   check_valid(desp);
+  if (desp.dtype_a == 3) {
+    // int32 epilogue arithmetic: alpha, beta and act_alpha become int32 by
+    // truncation, and sigmoid has no integer form
+    CS_ARG(p.act_type != 2, "sigmoid is not available for int8 GEMMs (int32 compute)");
+    for (float v : {p.alpha, p.beta, p.act_alpha})
+      CS_ARG(std::isfinite(v) && std::fabs(v) < 2147483648.0f, "alpha, beta and act_alpha of an int8 GEMM must fit int32");
+  }
   CS_RT(!empty(p.a) && !empty(p.b) && !empty(p.c), "a,b,c must not empty");
   if (desp.shuffle_type == 1) CS_RT(!empty(p.c_inds), "a_inds,c_inds tensor must not empty");
   if (desp.shuffle_type == 2) CS_RT(!empty(p.a_inds) && !empty(p.b_inds), "a_inds,b_inds tensor must not empty");
@@ -935,13 +942,6 @@ GemmPlan plan_gemm(const cs_gemm_params& p) {
   if (!empty(p.c_inds)) {
     check_index_dtype(p.c_inds, "c_inds");
     CS_RT(numel(p.c_inds) >= g.m, "c_inds must hold m entries");
-  }
-  if (desp.dtype_a == 3) {
-    // int32 epilogue arithmetic: alpha, beta and act_alpha become int32 by
-    // truncation, and sigmoid has no integer form
-    CS_ARG(p.act_type != 2, "sigmoid is not available for int8 GEMMs (int32 compute)");
-    for (float v : {p.alpha, p.beta, p.act_alpha})
-      CS_ARG(std::isfinite(v) && std::fabs(v) < 2147483648.0f, "alpha, beta and act_alpha of an int8 GEMM must fit int32");
   }
   const int es = dtype_bytes(p.a.dtype);
   const int vec = vec_elems(p.a.dtype);

@@ -106,14 +106,17 @@ _lib: Optional[C.CDLL] = None
 
 
 def lib() -> C.CDLL:
-    """Load libcumm_amd.so once; raise if it has not been built."""
+    """Load libcumm_amd.so once; raise if it has not been built.
+    ``CUMM_AMD_LIBRARY`` names another build of the same library (the
+    host-sanitizer build of ``_build.build_sanitized``)."""
     global _lib
     if _lib is not None:
         return _lib
-    if not LIB_PATH.exists():
-        raise RuntimeError(f"{LIB_PATH} is missing: build it with cumm_amd._build.build() "
+    path = Path(os.environ["CUMM_AMD_LIBRARY"]) if os.environ.get("CUMM_AMD_LIBRARY") else LIB_PATH
+    if not path.exists():
+        raise RuntimeError(f"{path} is missing: build it with cumm_amd._build.build() "
                            "(hipcc --offload-arch=gfx950)")
-    L = C.CDLL(str(LIB_PATH))
+    L = C.CDLL(str(path))
     L.cs_abi_version.restype = C.c_int
     L.cs_implicit_gemm2.argtypes = [C.POINTER(CsConvParams)]
     L.cs_implicit_gemm2.restype = C.c_int

@@ -63,5 +63,45 @@ def build(verbose: bool = False, jobs: int = 0) -> Path:
     return LIB
 
 
+# Host-sanitizer build: the host translation unit (abi.cpp: argument
+# validation, planning, workspace carving) with AddressSanitizer and
+# UndefinedBehaviorSanitizer, linked with the kernels' objects from build()
+# into _build/libcumm_amd_san.so.  Device code is never instrumented (each
+# -fsanitize= sits behind -Xarch_host).  Load it with CUMM_AMD_LIBRARY and the
+# ASan runtime preloaded (asan_runtime()); tests/test_sanitized_host.py does.
+SAN_LIB = BUILD / "libcumm_amd_san.so"
+SAN_FLAGS = [
+    "-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+    "-Xarch_host", "-fno-sanitize-recover=undefined", "-Xarch_host", "-fno-omit-frame-pointer",
+]
+
+
+def asan_runtime() -> Path:
+    """The clang ASan runtime matching hipcc (empty path if absent)."""
+    found = sorted(Path("/opt/rocm/llvm/lib/clang").glob("*/lib/linux/libclang_rt.asan-x86_64.so"))
+    return found[-1] if found else Path()
+
+
+def build_sanitized(verbose: bool = False) -> Path:
+    objs = [BUILD / (s.stem + ".o") for s in sources() if s.name != "abi.cpp"]
+    if not all(o.exists() for o in objs):
+        build(verbose=verbose)
+    san_obj = BUILD / "abi_san.o"
+    cmd = [HIPCC, *[f for f in COMMON if f != "-O3"], "-O1", "-g", *SAN_FLAGS, "-c", str(CSRC / "abi.cpp"),
+           "-o", str(san_obj)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for abi.cpp (sanitized):\n{r.stderr}")
+    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *SAN_FLAGS[:4], str(san_obj), *map(str, objs),
+           "-o", str(SAN_LIB)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"sanitized link failed:\n{r.stderr}")
+    if verbose:
+        print(f"built {SAN_LIB}", file=sys.stderr)
+    return SAN_LIB
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_sanitized(verbose=True)

@@ -330,3 +330,67 @@ def test_int8_inference_catalogue_and_validation():
         run(params(output_add=torch.zeros((100, 32), dtype=torch.float16)))
     with pytest.raises(ValueError, match="GPU"):
         run(params(output_add=torch.zeros((100, 32), dtype=torch.int8)))  # otherwise valid, host tensors
+
+
+def test_dynamic_mask_and_int8_gemm_validation():
+    """Host checks of the dynamic-mask word count, mask_int_count and the
+    int8 GEMM epilogue (no device work is reached)."""
+    run = ConvMainUnitTest.implicit_gemm2
+    N, C, K = 100, 16, 16
+    p = ConvParams(3, ConvOpType.Forward)
+    p.conv_algo_desp = select_conv_algo(0, tv.float16, tv.float16, tv.float16, C, K, True)
+    p.input = tv.from_torch(torch.zeros((N, C), dtype=torch.float16))
+    p.weight = tv.from_torch(torch.zeros((K, 343, C), dtype=torch.float16))
+    p.output = tv.from_torch(torch.zeros((N, K), dtype=torch.float16))
+    p.indices = tv.from_torch(torch.full((343, N), -1, dtype=torch.int32))
+    p.mask = tv.from_torch(torch.zeros((N, 11), dtype=torch.int32))
+    p.mask_argsort = tv.from_torch(torch.arange(N, dtype=torch.int32))
+    p.mask_int_count = 3
+    with pytest.raises(ValueError, match="mask_int_count"):
+        run(p)
+    p.mask_int_count = 11
+    with pytest.raises(ValueError, match="GPU"):
+        run(p)  # the word count checks pass; host tensors stop it
+    p.weight = tv.from_torch(torch.zeros((K, 1331, C), dtype=torch.float16))
+    p.indices = tv.from_torch(torch.full((1331, N), -1, dtype=torch.int32))
+    p.mask_int_count = 1
+    with pytest.raises(ValueError, match="up to 1024"):
+        run(p)  # 11 x 11 x 11
+    d = [x for x in GemmMainUnitTest.get_all_algo_desp() if x.dtype_a == tv.int8 and x.dtype_c == tv.int8
+         and x.shuffle_type == ShuffleStrideType.NoShuffle][0]
+    g = GemmParams()
+    g.algo_desp = d
+    g.a = tv.from_torch(torch.zeros((64, 32), dtype=torch.int8))
+    g.b = tv.from_torch(torch.zeros((32, 48) if not d.trans_b else (48, 32), dtype=torch.int8))
+    g.c = tv.from_torch(torch.zeros((64, 48), dtype=torch.int8))
+    g.act_type = Activation.Sigmoid
+    with pytest.raises(ValueError, match="sigmoid"):
+        GemmMainUnitTest.matmul2(g)
+    g.act_type = Activation.None_
+    g.alpha = float("inf")
+    with pytest.raises(ValueError, match="int32"):
+        GemmMainUnitTest.matmul2(g)
+
+
+def test_pair_geometry_validation():
+    """cs_pairs_*_geom refuse bad geometries before any device work."""
+    L = _abi.lib()
+    nb = ctypes.c_int64(0)
+
+    def geom(k=(3, 3, 3), s=(2, 2, 2), p=(1, 1, 1), d=(1, 1, 1)):
+        g = _abi.CsConvGeom()
+        for i in range(3):
+            g.ksize[i], g.stride[i], g.padding[i], g.dilation[i] = k[i], s[i], p[i], d[i]
+        return g
+
+    assert L.cs_pairs_subm_geom_workspace(1000, ctypes.byref(geom(k=(5, 5, 5))), ctypes.byref(nb)) == 0 and nb.value > 0
+    assert L.cs_pairs_subm_geom_workspace(1000, ctypes.byref(geom(k=(4, 3, 3))), ctypes.byref(nb)) == 1
+    assert b"odd" in L.cs_pairs_last_error()
+    assert L.cs_pairs_subm_geom_workspace(1000, ctypes.byref(geom(k=(11, 11, 11))), ctypes.byref(nb)) == 1
+    assert b"1024" in L.cs_pairs_last_error()
+    assert L.cs_pairs_subm_geom_workspace(1000, ctypes.byref(geom(d=(0, 1, 1))), ctypes.byref(nb)) == 1
+    assert L.cs_pairs_strided_geom_workspace(1000, 500, ctypes.byref(geom(k=(2, 2, 2), p=(0, 0, 0))),
+                                             ctypes.byref(nb)) == 0
+    assert L.cs_pairs_strided_geom_workspace(1000, 500, ctypes.byref(geom(s=(0, 2, 2))), ctypes.byref(nb)) == 1
+    assert L.cs_pairs_strided_out_geom_workspace(1000, ctypes.byref(geom(p=(-1, 0, 0))), 0, ctypes.byref(nb)) == 1
+    assert L.cs_pairs_subm_geom_workspace(1000, None, ctypes.byref(nb)) == 1
