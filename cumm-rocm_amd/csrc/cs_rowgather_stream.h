@@ -132,10 +132,12 @@ __global__ __launch_bounds__(64 * kStWaves) void rowgather_stream_kernel(RowGath
       }
     }
   };
-  // pair index of row lr for offset f (every lane group holds its row's copy)
-  auto load_idx = [&](int f) -> int {
-    if (f < 0 || !rvalid) return -1;
-    return p.pairs ? p.pairs[(int64_t)f * p.pair_ld + col] : col;
+  // pair index of row lr for window-local offset f (every lane group holds its
+  // row's copy); none is read for an offset this wave's tile lacks, whose
+  // MFMAs it skips
+  auto load_idx = [&](int base, int f) -> int {
+    if (f < 0 || !rvalid || !mask_bit(km, f)) return -1;
+    return p.pairs ? p.pairs[(int64_t)(base + f) * p.pair_ld + col] : col;
   };
   // gathered operand: lane (lr, lg) holds row lr, reduction elements
   // kc*CH + lg*E .. + E of each chunk kc
@@ -160,8 +162,8 @@ __global__ __launch_bounds__(64 * kStWaves) void rowgather_stream_kernel(RowGath
     const int f1 = next_offset(U, f0);
     u32x4 wv[WCH];
     load_w(base + f0, wv);
-    int i0 = load_idx(base + f0);
-    int i1 = load_idx(f1 >= 0 ? base + f1 : -1);
+    int i0 = load_idx(base, f0);
+    int i1 = load_idx(base, f1);
     u32x4 a_cur[KC];
     gather(i0, a_cur);
     store_w(0, wv);
@@ -173,7 +175,7 @@ __global__ __launch_bounds__(64 * kStWaves) void rowgather_stream_kernel(RowGath
       // rows of the next offset and pair indices of the one after
       u32x4 a_nxt[KC];
       gather(i1, a_nxt);
-      const int i2 = load_idx(fnn >= 0 ? base + fnn : -1);
+      const int i2 = load_idx(base, fnn);
       if (mask_bit(km, fc)) {
         const char* wb = wlds + buf * STAGE;
 #pragma unroll
