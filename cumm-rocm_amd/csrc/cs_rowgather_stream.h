@@ -48,7 +48,12 @@ __global__ __launch_bounds__(64 * kStWaves) void rowgather_stream_kernel(RowGath
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int lr = lane & 15, lg = lane >> 4;
-  const int t = blockIdx.x * kStWaves + wave;
+  // Rows are sorted by mask, and tiles late in that order carry more offsets
+  // (the high offsets are the sort's most significant bits), so blocks walk
+  // the tile groups from the last one back: the longest blocks start first
+  // and the short ones fill in behind them.
+  const int grp = (int)gridDim.x - 1 - (int)blockIdx.x;
+  const int t = grp * kStWaves + wave;
   const int row0 = t * kMaskTile;
   const int r = row0 + lr;
   const bool rvalid = t < ntiles && r < p.m;
@@ -239,6 +244,47 @@ __global__ __launch_bounds__(64 * kStWaves) void rowgather_stream_kernel(RowGath
   }
   OutT* Cout = reinterpret_cast<OutT*>(p.c);
   const OutT* D = reinterpret_cast<const OutT*>(p.d);
+  // Whole 16-byte row segments through a per-wave LDS tile (the weight
+  // stages are free once the offset walk is done): beta == 0, all columns
+  // inside nout, 16-byte aligned rows.
+  constexpr int ROWB = NCOL * (int)sizeof(OutT);
+  constexpr bool kStage = ROWB % 16 == 0 && kStWaves * 16 * ROWB <= kStLdsBytes;
+  const bool vec_store = kStage && p.beta == 0.f && NCOL <= p.nout &&
+                         ((int64_t)p.ldc * (int64_t)sizeof(OutT)) % 16 == 0 &&
+                         (reinterpret_cast<uintptr_t>(Cout) & 15u) == 0;
+  if (vec_store) {
+    char* stage = wlds + wave * 16 * ROWB;
+    OutT* st = reinterpret_cast<OutT*>(stage);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        const int n = nb * 16 + lr;
+        float v;
+        if constexpr (IsInt8<T>::value) {
+          v = i8_epilogue(p, __float_as_int(acc[nb][i]), ep_s[nb], ep_b[nb], n, 0);  // beta == 0 here
+        } else {
+          v = acc[nb][i];
+          if (p.scale) v *= ep_s[nb];
+          v = apply_act(v * p.alpha, p.act_type, p.act_alpha);
+        }
+        st[(4 * lg + i) * NCOL + n] = from_f32<OutT>(v);
+      }
+    __builtin_amdgcn_wave_barrier();
+    constexpr int LPR = ROWB / 16;  // lanes per row
+    constexpr int RPI = 64 / LPR;   // rows per store instruction
+#pragma unroll
+    for (int rb = 0; rb < 16; rb += RPI) {
+      const int rr = rb + lane / LPR;
+      const int chunk = lane % LPR;
+      const int o = __shfl(orow, rr & 15);
+      if (rr < 16 && row0 + rr < p.m) {
+        const u32x4 v = *reinterpret_cast<const u32x4*>(stage + rr * ROWB + chunk * 16);
+        *reinterpret_cast<u32x4*>(reinterpret_cast<char*>(Cout + (int64_t)o * p.ldc) + chunk * 16) = v;
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int rr = 4 * lg + i;
