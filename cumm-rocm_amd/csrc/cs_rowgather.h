@@ -48,15 +48,11 @@ __global__ __launch_bounds__(256) void rowgather_kernel(RowGatherArgs p) {
   __shared__ uint32_t tmw[MW > 1 ? 4 : 1][MW > 1 ? 4 : 1][MW > 1 ? kMaxDynWords : 1];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so give
-  // each XCD a contiguous range of row tiles (shared neighbours stay in one L2).
-  const int nblk = gridDim.x;
-  const int bid = blockIdx.x;
-  const int q = nblk >> 3, r8 = nblk & 7;
-  const int xcd = bid & 7;
-  const int slot = bid >> 3;
-  // XCDs below r8 get q + 1 consecutive tiles, the others q: a bijection on [0, nblk)
-  const int tile_blk = xcd < r8 ? xcd * (q + 1) + slot : r8 * (q + 1) + (xcd - r8) * q + slot;
+  // Rows are sorted by mask, and the blocks late in that order carry the most
+  // offsets (the high offsets are the sort's most significant bits): blocks
+  // take the row groups from the last one back, so the longest start first and
+  // every XCD (workgroups are dealt to the 8 XCDs in turn) gets an even share.
+  const int tile_blk = (int)gridDim.x - 1 - (int)blockIdx.x;
   const int row0 = tile_blk * kRgRows;
   if (row0 >= p.m) return;  // whole block leaves together
   const int n0 = blockIdx.y * (NB * 16);
