@@ -153,9 +153,8 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
     return (uint32_t)(((int64_t)f * p.pair_ld + col) * 4) | ((uint32_t)(f >> 31) << 31);
   };
   // next batch: the first BATCH offsets of rem (returned as a bit mask);
-  // pair indices issued (lane group lg loads slot lg, and lg + 4 when BATCH is
-  // 8), not waited on
-  auto batch = [&](uint32_t& rem, int col, int& ia, int& ib) -> uint32_t {
+  // pair indices issued (lane group lg loads slot lg), not waited on
+  auto batch = [&](uint32_t& rem, int col, int& ia) -> uint32_t {
     uint32_t bm = 0u;
 #pragma unroll
     for (int i = 0; i < BATCH; ++i) {
@@ -166,24 +165,20 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
     int fs[BATCH];
 #pragma unroll
     for (int i = 0; i < BATCH; ++i) fs[i] = slot_f(bm, i);
+    static_assert(BATCH == 2 || BATCH == 3, "one pair-index load per lane group");
     int fa;
-    if constexpr (BATCH >= 4) fa = lg == 0 ? fs[0] : lg == 1 ? fs[1] : lg == 2 ? fs[2] : fs[3];
-    else if constexpr (BATCH == 3) fa = lg == 0 ? fs[0] : lg == 1 ? fs[1] : lg == 2 ? fs[2] : -1;
-    else fa = lg == 0 ? fs[0] : lg == 1 ? fs[1 % BATCH] : -1;
-    int fb = -1;
-    if constexpr (BATCH == 8) fb = lg == 0 ? fs[4] : lg == 1 ? fs[5] : lg == 2 ? fs[6] : fs[7];
-    // unconditional buffer loads (slot past the batch: offset past the extent)
+    if constexpr (BATCH == 3) fa = lg == 0 ? fs[0] : lg == 1 ? fs[1] : lg == 2 ? fs[2] : -1;
+    else fa = lg == 0 ? fs[0] : lg == 1 ? fs[1] : -1;
+    // unconditional buffer load (slot past the batch: offset past the extent)
     ia = (int)__builtin_amdgcn_raw_buffer_load_b32(rpair, pair_off(fa, col), 0, 0);
-    ib = -1;
-    if constexpr (BATCH == 8) ib = (int)__builtin_amdgcn_raw_buffer_load_b32(rpair, pair_off(fb, col), 0, 0);
     return bm;
   };
-  // gathered rows of a batch: slot i's index for row lr sits in lane (i % 4) * 16 + lr
-  auto gather = [&](uint32_t bm, int ia, int ib, u32x4 (&af)[BATCH][KC]) {
+  // gathered rows of a batch: slot i's index for row lr sits in lane i * 16 + lr
+  auto gather = [&](uint32_t bm, int ia, u32x4 (&af)[BATCH][KC]) {
     const int cnt = __builtin_popcount(bm);
 #pragma unroll
     for (int i = 0; i < BATCH; ++i) {
-      int idx = __shfl(i < 4 ? ia : ib, (i & 3) * 16 + lr);
+      int idx = __shfl(ia, i * 16 + lr);
       idx = i < cnt ? idx : -1;
 #pragma unroll
       for (int kc = 0; kc < KC; ++kc) {
@@ -302,7 +297,7 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
   // stores wait for the weights only).
   bool live_c = false;
   uint32_t rem_c = 0u, bm_c = 0u;
-  int ia_c = -1, ib_c = -1;
+  int ia_c = -1;
   auto first_tile = [&]() {
     // order the pool: counting sort by offsets, descending
     if (ordered) {
@@ -357,7 +352,7 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
     first_meta(kWsWaves + wave, t1, Mn);
     __syncthreads();  // the stage buffers are free again
     rem_c = prepare(tile_mask(t0, Mc.m), live_c);  // empty past the last tile
-    bm_c = batch(rem_c, Mc.col, ia_c, ib_c);
+    bm_c = batch(rem_c, Mc.col, ia_c);
   };
   if (p.b_kmajor) {
     // B element (f, n, k) at b[k * b_ks + f * b_fs + n] (dgrad reads the KRSC
@@ -438,36 +433,35 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
   int tn = t1;
   while (true) {
     u32x4 af[BATCH][KC];
-    gather(bm_c, ia_c, ib_c, af);
+    gather(bm_c, ia_c, af);
     // next tile: its mask is final once its row data has landed
     const int tnn = tn < ntiles ? next_tile() : ntiles;
     bool live_n = false;
     uint32_t rem_n = 0u, bm_n = 0u;
-    int ia_n = -1, ib_n = -1;
+    int ia_n = -1;
     Meta Mnn;
     // unconditional: past the last tile the row data reads empty and the
     // batch is empty, so no load sits behind a branch
     rem_n = prepare(tile_mask(tn, Mn.m), live_n);
-    bm_n = batch(rem_n, Mn.col, ia_n, ib_n);
+    bm_n = batch(rem_n, Mn.col, ia_n);
     load_meta(tnn, Mnn);
     // a tile with more than BATCH offsets: the pair indices of each further
     // batch load while the rows of the batch before it are gathered, so every
     // further batch costs one round trip
     uint32_t bm_x = 0u;
-    int ia_x = -1, ib_x = -1;
-    bm_x = batch(rem_c, Mc.col, ia_x, ib_x);
+    int ia_x = -1;
+    bm_x = batch(rem_c, Mc.col, ia_x);
     f32x4 acc[NBC];
 #pragma unroll
     for (int nb = 0; nb < NBC; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
     mma_batch(bm_c, af, acc);
     while (bm_x) {
-      gather(bm_x, ia_x, ib_x, af);
-      int ia_y = -1, ib_y = -1;
-      const uint32_t bm_y = batch(rem_c, Mc.col, ia_y, ib_y);
+      gather(bm_x, ia_x, af);
+      int ia_y = -1;
+      const uint32_t bm_y = batch(rem_c, Mc.col, ia_y);
       mma_batch(bm_x, af, acc);
       bm_x = bm_y;
       ia_x = ia_y;
-      ib_x = ib_y;
     }
     if (live_c) epilogue(t, Mc.orow, acc);
     if (tn >= ntiles) break;
@@ -479,7 +473,6 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
     rem_c = rem_n;
     bm_c = bm_n;
     ia_c = ia_n;
-    ib_c = ib_n;
   }
 }
 
