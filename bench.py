@@ -228,6 +228,20 @@ def copy_calibration(nbytes: int):
     log(f"copy calibration: {nbytes} bytes x 3")
 
 
+def kernel_name(op: str, dt: str, C: int, K: int) -> str:
+    """The main kernel of a subm 3x3x3 op at these channels (the launcher's
+    choice for the default catalogue kernel): wgrad's partial products; for
+    forward / dgrad the weight-stationary row gather while the weights of all
+    offsets fit two column slices (four for FP32), the streamed-weights kernel
+    for wider 16-bit / fp8 layers, the block row gather for wide FP32 ones."""
+    if op == "wgrad":
+        return "wgrad_kernel"
+    nout = K if op == "fwd" else C
+    if nout <= 64:
+        return "rowgather_ws_kernel"
+    return "rowgather_kernel" if dt == "f32" else "rowgather_stream_kernel"
+
+
 def load_traffic(kernel: str, workload: str):
     """Measured HBM bytes per launch of `kernel` on `workload` from the counter
     summary (profiles/traffic.json), or None when none was taken for them."""
@@ -863,7 +877,7 @@ def run_single(args, with_cpu: bool) -> dict:
     byt = op_bytes(rop, prob.n, prob.kv, C, K, s, out_bytes(main_dt, rop))
     t_k = mean_op[rop] * 1e-3
     ach_bw = byt / t_k
-    kname = {"fwd": "rowgather_ws_kernel", "dgrad": "rowgather_ws_kernel", "wgrad": "wgrad_kernel"}[rop]
+    kname = kernel_name(rop, main_dt, C, K)
     tr = load_traffic(f"{kname}<{main_dt}> {rop}", workload)
     roofline = {"kernel": f"{kname}<{main_dt}> ({rop})", "desp": names[rop], "bound": "hbm",
                 "achieved": ach_bw / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach_bw / HBM_PEAK,
@@ -876,9 +890,12 @@ def run_single(args, with_cpu: bool) -> dict:
     roofline_ops = {}
     for k, v in mean_op.items():
         b = op_bytes(k, prob.n, prob.kv, C, K, s, out_bytes(main_dt, k))
+        trk = load_traffic(f"{kernel_name(k, main_dt, C, K)}<{main_dt}> {k}", workload)
         roofline_ops[k] = {"avg_us": v * 1e3, "algorithmic_bytes": b, "GB/s": b / (v * 1e-3) / 1e9,
                            "frac_hbm": b / (v * 1e-3) / HBM_PEAK,
-                           "frac_mfma": flops_op / (v * 1e-3) / PEAK[main_dt]}
+                           "frac_mfma": flops_op / (v * 1e-3) / PEAK[main_dt],
+                           "kernel": kernel_name(k, main_dt, C, K),
+                           "traffic": trk["traffic_bytes"] if trk else None}
 
     cpu = cpu_baseline(prob, X, W, dO, args.cpu_seconds) if with_cpu else None
     accuracy = accuracy_vs_fp64(prob, X, W, dO, gpu_outputs)
