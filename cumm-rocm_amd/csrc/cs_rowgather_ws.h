@@ -358,6 +358,9 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
     // B element (f, n, k) at b[k * b_ks + f * b_fs + n] (dgrad reads the KRSC
     // weights directly): a unit of E reduction rows x E columns loads as E
     // 16-byte rows and is transposed in registers into E chunks along k.
+    // Consecutive threads take consecutive column units of the same rows, so
+    // each load instruction reads whole row segments (the column units of a
+    // row are contiguous in memory; consecutive reduction units are not).
     constexpr int S = (int)sizeof(T);
     constexpr int NU = NCOL / E, KU = C / E;
     constexpr int UPT = (32 * NU * KU + 64 * kWsWaves - 1) / (64 * kWsWaves);  // kv <= 32
@@ -367,7 +370,7 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
 #pragma unroll
     for (int j = 0; j < UPT; ++j) {
       const int u = tid + j * (64 * kWsWaves);
-      const int f = u / (NU * KU), nu = (u / KU) % NU, ku = u % KU;
+      const int f = u / (NU * KU), ku = (u / NU) % KU, nu = u % NU;
       const bool ok = u < units && n0 + nu * E < p.nout;
 #pragma unroll
       for (int i = 0; i < E; ++i)
@@ -379,7 +382,7 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
     for (int j = 0; j < UPT; ++j) {
       const int u = tid + j * (64 * kWsWaves);
       if (u < units) {
-        const int f = u / (NU * KU), nu = (u / KU) % NU, ku = u % KU;
+        const int f = u / (NU * KU), ku = (u / NU) % KU, nu = u % NU;
 #pragma unroll
         for (int jn = 0; jn < E; ++jn) {
           u32x4 o = zero16();
