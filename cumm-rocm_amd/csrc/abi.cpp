@@ -520,10 +520,13 @@ int resolve_splits(int requested, int64_t M, int64_t N) {
 }
 
 // Lower bound of pairs per wgrad piece: pieces of an offset scale with its
-// pair count.  1024 pairs (16 stages of 64 rows at 64 channels) measured
-// best on MI355X for 64- and 128-channel layers: fewer pieces mean fewer
-// FP32 partials to write and reduce, more keep more blocks in flight.
+// pair count.  Fewer pieces mean fewer FP32 partials to write and reduce,
+// more keep more blocks in flight.  Measured on MI355X: 1024 pairs (16 stages
+// of 64 rows at 64 channels) for 64-wide output tiles; with 128 x 128 tiles a
+// partial is four times larger and 2048 pairs were 14-20 % faster for every
+// dtype at 200 000 voxels, 128 -> 128 (1024 and 4096 slower).
 constexpr int kWgradMinRows = 1024;
+int wgrad_min_rows(int64_t M, int64_t N) { return M > 64 && N > 64 ? 2 * kWgradMinRows : kWgradMinRows; }
 
 // wgrad workspace: FP32 partials, then (conv form only) the effective row
 // masks, per-block pair counts, the per-offset pair lists and their totals.
@@ -807,7 +810,7 @@ void run_conv(const cs_conv_params& p) {
     w.mask_filter = plan.mw > 1 ? p.mask_filter : p.mask_filter & (plan.kv >= 32 ? 0xffffffffu : ((1u << plan.kv) - 1u));
     w.mw = plan.mw;
     w.max_blocks = plan.splits;
-    w.min_rows = kWgradMinRows;
+    w.min_rows = wgrad_min_rows(plan.K, plan.C);
     w.fixed_total = -1;
     const WgradLayout lay = wgrad_layout(plan.kv, plan.K, plan.C, plan.N, plan.splits, true, plan.mw);
     char* base = static_cast<char*>(ws.ptr);
@@ -1216,7 +1219,7 @@ __attribute__((visibility("hidden"))) int implicit_gemm2_in(const cs_conv_params
     ConvPlan plan = plan_conv(q);
     if (plan.op == 2 && plan.workspace > cap) {
       // fewer split pieces per offset: the partition only changes for an
-      // offset with more than pieces * kWgradMinRows pairs
+      // offset with more than pieces * wgrad_min_rows(K, C) pairs
       const int64_t other = wgrad_layout(plan.kv, plan.K, plan.C, plan.N, 0, true, plan.mw).size;
       const int64_t per = plan.kv * plan.K * plan.C * 4;
       const int64_t fit = (cap - other - 256) / std::max<int64_t>(per, 1);
