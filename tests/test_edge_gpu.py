@@ -240,3 +240,29 @@ def test_wgrad_epilogue(case, dt, out_dt):
         exp = ref.conv_wgrad(xq, dq, prob.pair_fwd, tm, prob.argsort, 16, alpha=alpha, beta=beta, dW_init=init,
                              act_type=act, act_alpha=aa, bias=bias if use_bias else None)
         assert _rel(dw.double().cpu().numpy(), quantize(exp, out_dt)) <= 2 * max(TOL[dt], TOL[out_dt])
+
+
+@pytest.mark.parametrize("K", [64, 128])
+@pytest.mark.parametrize("dt", ["f16", "bf16"])
+def test_streamed_forward_staged_epilogue(dt, K):
+    """beta = 0 with alpha and an activation on the streamed-weights kernel
+    (tile_shape[1] = 128): output rows leave through its LDS stage as whole
+    16-byte segments (64- and 128-column blocks), tile groups run from the
+    last one back; mask_output stays bit-exact."""
+    c = subm_case(3000, 64, K, seed=K + 7)
+    prob = c.prob
+    xq, wq = quantize(c.X, dt), quantize(c.W, dt)
+    tiles = (prob.n + 15) // 16
+    for alpha, act, aa in ((0.5, 1, 0.0), (1.25, 3, 0.1), (1.0, 2, 0.0)):
+        print(f"[stream-epi] dt={dt} K={K} alpha={alpha} act={act}", flush=True)
+        out = torch.full((prob.n, K), 7.0, dtype=TORCH_DT[dt], device="cuda")
+        mo = torch.zeros(tiles, dtype=torch.int32, device="cuda")
+        p = conv_params(0, _desp(0, dt, 64, K, 128), to_dev(c.X, dt), to_dev(c.W, dt), out, prob.pair_fwd,
+                        prob.mask_sorted, prob.argsort, mask_output=mo, alpha=alpha, beta=0.0, act_type=act,
+                        act_alpha=aa)
+        ConvMainUnitTest.implicit_gemm2(p)
+        torch.cuda.synchronize()
+        exp, tm = ref.conv_forward(xq, wq, prob.pair_fwd, prob.mask_sorted, prob.argsort, 16, alpha=alpha,
+                                   act_type=act, act_alpha=aa)
+        assert np.array_equal(mo.cpu().numpy().view(np.uint32), tm), "mask_output must be bit-exact"
+        assert _rel(out.double().cpu().numpy(), quantize(exp, dt)) <= TOL[dt]
