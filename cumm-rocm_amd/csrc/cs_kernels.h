@@ -68,7 +68,7 @@ struct WgradArgs {
   int32_t mw;                 // mask words per group / row (1, or div_up(kv, 32) for dynamic masks)
   int32_t nrows, M, N, kv;
   uint32_t mask_filter;
-  int32_t max_blocks;        // pieces per offset (grid.z), <= kWgradMaxBlocks
+  int32_t max_blocks;        // pieces per offset, upper bound, <= kWgradMaxBlocks
   int32_t min_rows;          // pairs per piece, lower bound
   int32_t fixed_total;       // list form: entries of the given lists; else -1
   uint32_t* rowmask;         // [nrows][mw]
@@ -162,5 +162,6 @@ constexpr int kMaxDynWords = 32;     // dynamic masks: words per row, kv <= 1024
 constexpr int kWgradRows = 64;       // rows staged per wgrad step
 constexpr int kWgradMaxBlocks = 128; // pieces per offset, upper bound
 constexpr int kWgradTile = 64;       // wgrad output tile (M and N)
+constexpr int kWgradXcds = 8;        // XCDs of an MI355X: wgrad pieces are grouped per XCD
 
 }  // namespace cs

@@ -275,17 +275,49 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void w
   __shared__ __attribute__((aligned(16))) char lds_a[R * PA];
   __shared__ __attribute__((aligned(16))) char lds_b[R * PB];
 
-  const int f = blockIdx.y;
-  if (!((p.mask_filter >> (f & 31)) & 1u)) return;
-  const int count = wgrad_total(p, f);
-  const int nb = wgrad_pieces(count, p.max_blocks, p.min_rows);
-  const int piece = blockIdx.z;
-  if (piece >= nb) return;
-  const int tiles_n = (p.N + TN - 1) / TN;
-  const int m0 = (blockIdx.x / tiles_n) * TM;
-  const int nn0 = (blockIdx.x % tiles_n) * TN;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
+  const int tiles_n = (p.N + TN - 1) / TN;
+  const int tiles = ((p.M + TM - 1) / TM) * tiles_n;
+  // Block -> (offset, piece, tile).  Workgroups are dealt to the XCDs in turn
+  // (block b runs on XCD b % kWgradXcds), and each XCD takes the same eighth
+  // of every offset's pieces: offset f's lists run in ascending output row,
+  // so an XCD's blocks all gather rows of one eighth of the output rows and
+  // their neighbours, and a row needed by several offsets is read from that
+  // XCD's L2 after its first use.  Slot s of an XCD is found by a 64-lane
+  // scan over the offsets' piece counts (in offset order, tiles innermost).
+  const int xcd = (int)blockIdx.x % kWgradXcds, slot = (int)blockIdx.x / kWgradXcds;
+  int f = -1, piece = 0, tile = 0;
+  for (int f0 = 0, base = 0; f0 < p.kv && f < 0; f0 += 64) {
+    const int ff = f0 + lane;
+    int cnt = 0, lo = 0;
+    if (ff < p.kv && ((p.mask_filter >> (ff & 31)) & 1u)) {
+      const int nbf = wgrad_pieces(wgrad_total(p, ff), p.max_blocks, p.min_rows);
+      lo = nbf * xcd / kWgradXcds;
+      cnt = (nbf * (xcd + 1) / kWgradXcds - lo) * tiles;
+    }
+    int incl = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int t = __shfl_up(incl, d);
+      if (lane >= d) incl += t;
+    }
+    const int rel = slot - base - (incl - cnt);
+    const uint64_t hit = __ballot(cnt > 0 && rel >= 0 && rel < cnt);
+    if (hit) {
+      const int k = __builtin_ctzll(hit);
+      const int r = __shfl(rel, k);
+      f = f0 + k;
+      piece = __shfl(lo, k) + r / tiles;
+      tile = r % tiles;
+    }
+    base += __shfl(incl, 63);
+  }
+  if (f < 0) return;
+  const int count = wgrad_total(p, f);
+  const int nb = wgrad_pieces(count, p.max_blocks, p.min_rows);
+  const int m0 = (tile / tiles_n) * TM;
+  const int nn0 = (tile % tiles_n) * TN;
   const int wm = (wave & 1) * (TM / 2), wn = (wave >> 1) * (TN / 2);
   const int per = (count + nb - 1) / nb;
   const int q0 = piece * per;
@@ -645,7 +677,9 @@ hipError_t launch_wgrad(int in_dtype, const WgradArgs& a, hipStream_t s) {
   const bool few = a.fixed_total >= 0 && a.kv == 1 && pieces < 256;
   const int tm = a.M > 64 && !few ? 128 : 64, tn = a.N > 64 && !few ? 128 : 64;
   const int tiles = ((a.M + tm - 1) / tm) * ((a.N + tn - 1) / tn);
-  dim3 grid(tiles, a.kv, a.max_blocks);
+  // one-dimensional grid: an XCD takes at most ceil(max_blocks / kWgradXcds)
+  // pieces of each offset per tile (blocks past its share return at once)
+  dim3 grid((unsigned)(kWgradXcds * tiles * a.kv * ((a.max_blocks + kWgradXcds - 1) / kWgradXcds)), 1, 1);
   dim3 block(256, 1, 1);
   if (tiles == 0) return hipSuccess;
   switch (in_dtype) {
