@@ -24,7 +24,7 @@ def _coords(sizes, grid, seed):
 
 
 @pytest.mark.parametrize("sizes,grid", [([1], (4, 8, 8)), ([300], (6, 20, 20)), ([5000, 1200, 800], (16, 96, 96)),
-                                        ([20000], pc.GRID_KITTI)])
+                                        ([20000], pc.GRID_KITTI), ([150000], pc.GRID_KITTI)])
 def test_subm_tables_match_host(sizes, grid):
     print(f"[pairs] subm {sizes} {grid}", flush=True)
     c = _coords(sizes, grid, 3)
