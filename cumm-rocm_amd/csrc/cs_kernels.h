@@ -78,6 +78,7 @@ struct WgradArgs {
   int64_t list_ld;
   int32_t* total;            // [kv]
   float* partial;            // [kv][max_blocks][M][N] FP32
+  int32_t xcd_grid;          // set by launch_wgrad: partial products on a grid grouped per XCD (kv <= 64)
 };
 
 // out[mm*o_ms + f*o_fs + nn] = act(alpha * tree_sum_b partial[f][b][mm][nn] + beta * src)

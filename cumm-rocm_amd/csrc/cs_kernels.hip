@@ -279,20 +279,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void w
   const int lane = tid & 63, wave = tid >> 6;
   const int tiles_n = (p.N + TN - 1) / TN;
   const int tiles = ((p.M + TM - 1) / TM) * tiles_n;
-  // Block -> (offset, piece, tile).  Workgroups are dealt to the XCDs in turn
-  // (block b runs on XCD b % kWgradXcds), and each XCD takes the same eighth
-  // of every offset's pieces: offset f's lists run in ascending output row,
-  // so an XCD's blocks all gather rows of one eighth of the output rows and
-  // their neighbours, and a row needed by several offsets is read from that
-  // XCD's L2 after its first use.  Slot s of an XCD is found by a 64-lane
-  // scan over the offsets' piece counts (in offset order, tiles innermost).
-  const int xcd = (int)blockIdx.x % kWgradXcds, slot = (int)blockIdx.x / kWgradXcds;
+  // Block -> (offset, piece, tile).
   int f = -1, piece = 0, tile = 0;
-  for (int f0 = 0, base = 0; f0 < p.kv && f < 0; f0 += 64) {
-    const int ff = f0 + lane;
+  if (p.xcd_grid) {
+    // kv <= 64, one-dimensional grid.  Workgroups are dealt to the XCDs in
+    // turn (block b runs on XCD b % kWgradXcds), and each XCD takes the same
+    // eighth of every offset's pieces: offset f's lists run in ascending
+    // output row, so an XCD's blocks all gather rows of one eighth of the
+    // output rows and their neighbours, and a row needed by several offsets
+    // is read from that XCD's L2 after its first use.  Slot s of an XCD is
+    // found by a 64-lane scan over the offsets' piece counts (in offset
+    // order, tiles innermost).
+    const int xcd = (int)blockIdx.x % kWgradXcds, slot = (int)blockIdx.x / kWgradXcds;
     int cnt = 0, lo = 0;
-    if (ff < p.kv && ((p.mask_filter >> (ff & 31)) & 1u)) {
-      const int nbf = wgrad_pieces(wgrad_total(p, ff), p.max_blocks, p.min_rows);
+    if (lane < p.kv && ((p.mask_filter >> (lane & 31)) & 1u)) {
+      const int nbf = wgrad_pieces(wgrad_total(p, lane), p.max_blocks, p.min_rows);
       lo = nbf * xcd / kWgradXcds;
       cnt = (nbf * (xcd + 1) / kWgradXcds - lo) * tiles;
     }
@@ -302,18 +303,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void w
       const int t = __shfl_up(incl, d);
       if (lane >= d) incl += t;
     }
-    const int rel = slot - base - (incl - cnt);
+    const int rel = slot - (incl - cnt);
     const uint64_t hit = __ballot(cnt > 0 && rel >= 0 && rel < cnt);
-    if (hit) {
-      const int k = __builtin_ctzll(hit);
-      const int r = __shfl(rel, k);
-      f = f0 + k;
-      piece = __shfl(lo, k) + r / tiles;
-      tile = r % tiles;
-    }
-    base += __shfl(incl, 63);
+    if (!hit) return;
+    const int k = __builtin_ctzll(hit);
+    const int r = __shfl(rel, k);
+    f = k;
+    piece = __shfl(lo, k) + r / tiles;
+    tile = r % tiles;
+  } else {
+    // offset-major grid (tiles, kv, max_blocks): larger kernels, where a
+    // per-block scan over every offset would cost more than it saves
+    f = blockIdx.y;
+    piece = blockIdx.z;
+    tile = blockIdx.x;
+    if (!((p.mask_filter >> (f & 31)) & 1u)) return;
+    if (piece >= wgrad_pieces(wgrad_total(p, f), p.max_blocks, p.min_rows)) return;
   }
-  if (f < 0) return;
   const int count = wgrad_total(p, f);
   const int nb = wgrad_pieces(count, p.max_blocks, p.min_rows);
   const int m0 = (tile / tiles_n) * TM;
@@ -677,17 +683,22 @@ hipError_t launch_wgrad(int in_dtype, const WgradArgs& a, hipStream_t s) {
   const bool few = a.fixed_total >= 0 && a.kv == 1 && pieces < 256;
   const int tm = a.M > 64 && !few ? 128 : 64, tn = a.N > 64 && !few ? 128 : 64;
   const int tiles = ((a.M + tm - 1) / tm) * ((a.N + tn - 1) / tn);
-  // one-dimensional grid: an XCD takes at most ceil(max_blocks / kWgradXcds)
-  // pieces of each offset per tile (blocks past its share return at once)
-  dim3 grid((unsigned)(kWgradXcds * tiles * a.kv * ((a.max_blocks + kWgradXcds - 1) / kWgradXcds)), 1, 1);
+  // kv <= 64: one-dimensional grid grouped per XCD, where an XCD takes at
+  // most ceil(max_blocks / kWgradXcds) pieces of each offset per tile (blocks
+  // past its share return at once); else (tiles, kv, pieces)
+  WgradArgs w = a;
+  w.xcd_grid = a.kv <= 64;
+  dim3 grid = w.xcd_grid
+                  ? dim3((unsigned)(kWgradXcds * tiles * a.kv * ((a.max_blocks + kWgradXcds - 1) / kWgradXcds)), 1, 1)
+                  : dim3((unsigned)tiles, (unsigned)a.kv, (unsigned)a.max_blocks);
   dim3 block(256, 1, 1);
   if (tiles == 0) return hipSuccess;
   switch (in_dtype) {
-    case kF32: return launch_wgrad_t<float>(tm, tn, grid, a, s);
-    case kF16: return launch_wgrad_t<_Float16>(tm, tn, grid, a, s);
-    case kBF16: return launch_wgrad_t<bf16>(tm, tn, grid, a, s);
-    case kE4M3: return launch_wgrad_t<e4m3>(tm, tn, grid, a, s);
-    case kI8: return launch_wgrad_t<int8_t>(tm, tn, grid, a, s);
+    case kF32: return launch_wgrad_t<float>(tm, tn, grid, w, s);
+    case kF16: return launch_wgrad_t<_Float16>(tm, tn, grid, w, s);
+    case kBF16: return launch_wgrad_t<bf16>(tm, tn, grid, w, s);
+    case kE4M3: return launch_wgrad_t<e4m3>(tm, tn, grid, w, s);
+    case kI8: return launch_wgrad_t<int8_t>(tm, tn, grid, w, s);
     default: return hipErrorInvalidValue;
   }
 }
