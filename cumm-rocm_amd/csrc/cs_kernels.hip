@@ -686,11 +686,16 @@ hipError_t launch_wgrad(int in_dtype, const WgradArgs& a, hipStream_t s) {
   // kv <= 64: one-dimensional grid grouped per XCD, where an XCD takes at
   // most ceil(max_blocks / kWgradXcds) pieces of each offset per tile (blocks
   // past its share return at once); else (tiles, kv, pieces)
+  // conv form: an offset's list holds at most one pair per output row, so no
+  // offset has more than ceil(nrows / min_rows) pieces; the grid stops there
+  // (the partition, which uses max_blocks, is unchanged)
+  int mb = a.max_blocks;
+  if (a.fixed_total < 0)
+    mb = (int)std::min<int64_t>(mb, std::max<int64_t>(1, ((int64_t)a.nrows + a.min_rows - 1) / a.min_rows));
   WgradArgs w = a;
   w.xcd_grid = a.kv <= 64;
-  dim3 grid = w.xcd_grid
-                  ? dim3((unsigned)(kWgradXcds * tiles * a.kv * ((a.max_blocks + kWgradXcds - 1) / kWgradXcds)), 1, 1)
-                  : dim3((unsigned)tiles, (unsigned)a.kv, (unsigned)a.max_blocks);
+  dim3 grid = w.xcd_grid ? dim3((unsigned)(kWgradXcds * tiles * a.kv * ((mb + kWgradXcds - 1) / kWgradXcds)), 1, 1)
+                         : dim3((unsigned)tiles, (unsigned)a.kv, (unsigned)mb);
   dim3 block(256, 1, 1);
   if (tiles == 0) return hipSuccess;
   switch (in_dtype) {
