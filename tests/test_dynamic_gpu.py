@@ -95,6 +95,34 @@ def test_dynamic_forward_dgrad_wgrad(C, K, dt, ks):
     assert _rel(dw.double().cpu().numpy(), quantize(exp, dt)) <= 2 * TOL[dt]
 
 
+@pytest.mark.parametrize("dt", ["f16", "f32"])
+def test_two_word_masks_forward_wgrad(dt):
+    """5x3x3 kernel (kv = 45: two mask words, at most 64 offsets): the forward's
+    two-word tile masks, then wgrad, whose partial products run on the grid
+    grouped per XCD (kernels of up to 64 offsets), against the FP64 oracle."""
+    print(f"[dyn] 5x3x3 dt={dt}", flush=True)
+    coords = pc.plane_scene(3000, grid=GRID, seed=11)
+    prob = pc.build_subm(coords, GRID, ksize=(5, 3, 3))
+    assert prob.kv == 45 and prob.mask_sorted.shape == (prob.n, 2)
+    C, K = 64, 48
+    rng = np.random.default_rng(45)
+    X = rng.standard_normal((prob.n, C)).astype(np.float32)
+    W = (rng.standard_normal((K, prob.kv, C)) / np.sqrt(prob.kv * C)).astype(np.float32)
+    dO = rng.standard_normal((prob.n, K)).astype(np.float32)
+    xq, wq, dq = quantize(X, dt), quantize(W, dt), quantize(dO, dt)
+    got, mo = _fwd(prob, X, W, dt)
+    exp, tm = ref.conv_forward(xq, wq, prob.pair_fwd, prob.mask_sorted, prob.argsort, 16)
+    assert np.array_equal(mo, tm)
+    assert _rel(got, quantize(exp, dt)) <= TOL[dt]
+    wdesp = desp_for(2, dt, C, K, dynamic=True)
+    dw = torch.zeros((K, prob.kv, C), dtype=TORCH_DT[dt], device="cuda")
+    p = conv_params(2, wdesp, to_dev(X, dt), dw, to_dev(dO, dt), prob.pair_fwd, mo, prob.argsort, mask_width=16)
+    ConvMainUnitTest.implicit_gemm2(p)
+    torch.cuda.synchronize()
+    exp = ref.conv_wgrad(xq, dq, prob.pair_fwd, mo, prob.argsort, 16)
+    assert _rel(dw.double().cpu().numpy(), quantize(exp, dt)) <= 2 * TOL[dt]
+
+
 def test_dynamic_mask_filter_every_word():
     print("[dyn] mask filter", flush=True)
     prob, X, W, dO = _case5(2000, 32, 32, seed=5)

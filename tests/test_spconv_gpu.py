@@ -311,9 +311,12 @@ def test_streamed_equals_weight_stationary(n):
     assert torch.equal(outs[0], outs[1])
 
 
-@pytest.mark.parametrize("C,K,dt", [(128, 128, "f16"), (64, 128, "bf16"), (128, 64, "f32"), (128, 128, "e4m3")])
+@pytest.mark.parametrize("C,K,dt", [(128, 128, "f16"), (64, 128, "bf16"), (128, 64, "f32"), (128, 128, "e4m3"),
+                                    (256, 256, "f16"), (192, 320, "bf16")])
 def test_wgrad_wide_tiles(C, K, dt):
-    """wgrad with 128-wide output tiles (each gathered row serves 128 channels)."""
+    """wgrad with 128-wide output tiles (each gathered row serves 128 channels);
+    256 -> 256 and 192 -> 320 take several output tiles per piece (4 and 6), so
+    the per-XCD block map runs with tiles innermost."""
     c = subm_case(2000, C, K, seed=C + 2 * K)
     prob = c.prob
     print(f"[wgrad-wide] C={C} K={K} dt={dt}", flush=True)
