@@ -341,6 +341,7 @@ int subm_impl(const int32_t* coords, int64_t n, const int32_t* shape, const Geom
   hipLaunchKernelGGL(subm_lookup, dim3((unsigned)((n + 255) / 256), (unsigned)g.kv()), dim3(256), 0, s, coords, n,
                      sh, g, tkeys, tvals, cap - 1, pair_fwd, pair_bwd);
   PCHK(hipGetLastError());
+  if (mask_sorted == nullptr) return 0;  // pair tables only (row masks from cs_pairs_row_masks)
   return sort_masks(pair_fwd, n, g.kv(), mask_sorted, argsort, cv, s);
 }
 
@@ -417,7 +418,8 @@ int strided_impl(const int32_t* in_coords, int64_t n_in, const int32_t* in_shape
     PCHK(hipGetLastError());
   }
   const int64_t mark = cv.off;
-  int r = sort_masks(pair_fwd, n_out, kv, mfs, afs, cv, s);
+  // mfs null: pair tables only (row masks from cs_pairs_row_masks)
+  int r = mfs != nullptr ? sort_masks(pair_fwd, n_out, kv, mfs, afs, cv, s) : 0;
   if (r || pair_bwd == nullptr) return r;  // forward-only tables (inference)
   cv.off = mark;  // the backward sort reuses the scratch; both are ordered on one stream
   return sort_masks(pair_bwd, n_in, kv, mbs, abs_, cv, s);
@@ -487,7 +489,9 @@ int cs_pairs_subm_geom(const int32_t* coords, int64_t n, const int32_t* spatial_
   int64_t need = 0;
   if (cs_pairs_subm_geom_workspace(n, geom, &need)) return 1;
   if (n > 0x7fffffff || cs::bad_shape(spatial_shape)) return cs::fail("bad size or spatial shape");
-  if (ws_bytes < need || (n > 0 && (!coords || !pair_fwd || !mask_sorted || !argsort || !workspace)))
+  // mask_sorted and argsort are both set (sorted masks) or both null (pair tables only)
+  if ((mask_sorted == nullptr) != (argsort == nullptr)) return cs::fail("mask_sorted and argsort must be both set or both null");
+  if (ws_bytes < need || (n > 0 && (!coords || !pair_fwd || !workspace)))
     return cs::fail("null buffer or workspace too small");
   cs::Carver cv{static_cast<char*>(workspace)};
   return cs::subm_impl(coords, n, spatial_shape, cs::geom_of(geom), pair_fwd, pair_bwd, mask_sorted, argsort, cv,
@@ -570,12 +574,32 @@ int cs_pairs_strided_geom(const int32_t* in_coords, int64_t n_in, const int32_t*
   const bool bwd = pair_bwd != nullptr;
   if (bwd != (mask_bwd_sorted != nullptr) || bwd != (argsort_bwd != nullptr))
     return cs::fail("backward tables must be all set or all null");
-  if (ws_bytes < need || !workspace || (n_out > 0 && (!out_coords || !pair_fwd || !mask_fwd_sorted || !argsort_fwd)) ||
-      (n_in > 0 && !in_coords))
+  // the forward masks are sorted (both set) or left to cs_pairs_row_masks (both null)
+  if ((mask_fwd_sorted == nullptr) != (argsort_fwd == nullptr))
+    return cs::fail("mask_fwd_sorted and argsort_fwd must be both set or both null");
+  if (ws_bytes < need || !workspace || (n_out > 0 && (!out_coords || !pair_fwd)) || (n_in > 0 && !in_coords))
     return cs::fail("null buffer or workspace too small");
   cs::Carver cv{static_cast<char*>(workspace)};
   return cs::strided_impl(in_coords, n_in, in_shape, g, out_coords, n_out, pair_fwd, pair_bwd, mask_fwd_sorted,
                           argsort_fwd, mask_bwd_sorted, argsort_bwd, cv, reinterpret_cast<hipStream_t>(stream));
+}
+
+int cs_pairs_row_masks(const int32_t* pair, int64_t n, int32_t kv, uint32_t* mask, int32_t* identity,
+                       uint64_t stream) {
+  if (n < 0 || kv < 1 || kv > cs::kMaxKv || (n > 0 && (pair == nullptr || mask == nullptr)))
+    return cs::fail("bad argument");
+  if (n == 0) return 0;
+  if (n > 0x7fffffff / kv) return cs::fail("bad size");
+  const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(cs::row_masks, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pair, n, kv, mask);
+  const hipError_t em = hipGetLastError();
+  if (em != hipSuccess) return cs::fail("row_masks launch failed", em);
+  if (identity != nullptr) {
+    hipLaunchKernelGGL(cs::iota_i32, cs::grid1(n), dim3(256), 0, s, identity, n);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return cs::fail("iota launch failed", e);
+  }
+  return 0;
 }
 
 int cs_pairs_strided_workspace(int64_t n_in, int64_t n_out, int64_t* nbytes) {

@@ -30,7 +30,7 @@ SYMBOLS = [
     "cs_pairs_subm_workspace", "cs_pairs_subm", "cs_pairs_strided_out_workspace", "cs_pairs_strided_out",
     "cs_pairs_strided_workspace", "cs_pairs_strided", "cs_pairs_last_error",
     "cs_pairs_subm_geom_workspace", "cs_pairs_subm_geom", "cs_pairs_strided_out_geom_workspace",
-    "cs_pairs_strided_out_geom", "cs_pairs_strided_geom_workspace", "cs_pairs_strided_geom",
+    "cs_pairs_strided_out_geom", "cs_pairs_strided_geom_workspace", "cs_pairs_strided_geom", "cs_pairs_row_masks",
     "cs_get_all_gather_params", "cs_get_all_scatter_params", "cs_gather_supported", "cs_scatter_supported",
     "cs_gather", "cs_scatter",
 ]
@@ -154,10 +154,12 @@ def lib() -> C.CDLL:
     L.cs_pairs_strided_out_geom.argtypes = [P, I64, P, G, I32, P, I64, C.POINTER(I64), P, I64, U64]
     L.cs_pairs_strided_geom_workspace.argtypes = [I64, I64, G, C.POINTER(I64)]
     L.cs_pairs_strided_geom.argtypes = [P, I64, P, G, P, I64, P, P, P, P, P, P, P, I64, U64]
+    L.cs_pairs_row_masks.argtypes = [P, I64, I32, P, P, U64]
     for name in ("cs_pairs_subm_workspace", "cs_pairs_subm", "cs_pairs_strided_out_workspace",
                  "cs_pairs_strided_out", "cs_pairs_strided_workspace", "cs_pairs_strided",
                  "cs_pairs_subm_geom_workspace", "cs_pairs_subm_geom", "cs_pairs_strided_out_geom_workspace",
-                 "cs_pairs_strided_out_geom", "cs_pairs_strided_geom_workspace", "cs_pairs_strided_geom"):
+                 "cs_pairs_strided_out_geom", "cs_pairs_strided_geom_workspace", "cs_pairs_strided_geom",
+                 "cs_pairs_row_masks"):
         getattr(L, name).restype = C.c_int
     L.cs_pairs_last_error.restype = C.c_char_p
     L.cs_get_all_gather_params.argtypes = [P, C.c_int]

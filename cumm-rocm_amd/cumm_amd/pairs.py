@@ -93,7 +93,7 @@ class DeviceTables:
     pair_fwd: torch.Tensor         # [kv, N_out] int32
     pair_bwd: Optional[torch.Tensor]  # [kv, N_in] int32 (None: forward-only tables)
     mask_fwd: torch.Tensor         # [N_out] (or [N_out, W]) int32 view of uint32 masks, sorted
-    argsort_fwd: torch.Tensor
+    argsort_fwd: torch.Tensor      # (or, sort_masks=False: masks in row order, identity argsort)
     mask_bwd: Optional[torch.Tensor]
     argsort_bwd: Optional[torch.Tensor]
 
@@ -146,10 +146,21 @@ def _mask_buf(n: int, kv: int, dev) -> torch.Tensor:
     return torch.empty((n,) if w == 1 else (n, w), dtype=torch.int32, device=dev)
 
 
+def _row_masks(pf: torch.Tensor, ms: torch.Tensor, am: torch.Tensor, s: int) -> None:
+    """Masks of pair table pf in row order and the identity argsort (queued
+    on stream s)."""
+    kv, n = pf.shape
+    _check(_abi.lib().cs_pairs_row_masks(pf.data_ptr(), n, kv, ms.data_ptr(), am.data_ptr(), s))
+
+
 def subm_tables(coords: torch.Tensor, shape: Sequence[int], stream: Optional[int] = None,
-                backward: bool = True, ksize=(3, 3, 3), dilation=(1, 1, 1)) -> DeviceTables:
+                backward: bool = True, ksize=(3, 3, 3), dilation=(1, 1, 1), sort_masks: bool = True) -> DeviceTables:
     """``backward=False`` builds forward-only tables (no pair_bwd), all that
-    inference needs."""
+    inference needs.  ``sort_masks=False`` (forward-only tables) leaves the
+    masks in row order with the identity as argsort: implicit_gemm2 gives every
+    row the same result over unsorted masks (the tile grouping only changes how
+    dense the tile masks are), and the mask sort's time is saved."""
+    assert sort_masks or not backward, "unsorted masks are for forward-only tables"
     L = _abi.lib()
     spec = LayerSpec("subm", _triple(ksize), dilation=_triple(dilation))
     g = spec.cs()
@@ -168,7 +179,10 @@ def subm_tables(coords: torch.Tensor, shape: Sequence[int], stream: Optional[int
         _check(L.cs_pairs_subm_geom_workspace(n, C.byref(g), C.byref(nb)))
         ws = _ws(nb.value, dev)
         _check(L.cs_pairs_subm_geom(coords.data_ptr(), n, _shape_ptr(shape), C.byref(g), pf.data_ptr(), _ptr(pb),
-                                    ms.data_ptr(), am.data_ptr(), ws.data_ptr(), ws.numel(), s))
+                                    ms.data_ptr() if sort_masks else None, am.data_ptr() if sort_masks else None,
+                                    ws.data_ptr(), ws.numel(), s))
+        if not sort_masks:
+            _row_masks(pf, ms, am, s)
     shp = tuple(int(v) for v in shape)
     return DeviceTables("subm", coords, coords, shp, shp, pf, pb, ms, am, None, None)
 
@@ -202,8 +216,10 @@ def strided_out_coords(coords: torch.Tensor, shape: Sequence[int], rule: str = "
 
 def strided_tables(coords: torch.Tensor, shape: Sequence[int], rule: str = "down",
                    stream: Optional[int] = None, backward: bool = True, ksize=(3, 3, 3), stride=(2, 2, 2),
-                   padding=(1, 1, 1)) -> DeviceTables:
-    """``backward=False`` skips pair_bwd and the backward masks / argsort."""
+                   padding=(1, 1, 1), sort_masks: bool = True) -> DeviceTables:
+    """``backward=False`` skips pair_bwd and the backward masks / argsort;
+    ``sort_masks=False`` as for subm_tables."""
+    assert sort_masks or not backward, "unsorted masks are for forward-only tables"
     L = _abi.lib()
     spec = LayerSpec("strided", _triple(ksize), _triple(stride), _triple(padding))
     g = spec.cs()
@@ -224,22 +240,25 @@ def strided_tables(coords: torch.Tensor, shape: Sequence[int], rule: str = "down
         _check(L.cs_pairs_strided_geom_workspace(n_in, n_out, C.byref(g), C.byref(nb)))
         ws = _ws(nb.value, dev)
         _check(L.cs_pairs_strided_geom(coords.data_ptr(), n_in, _shape_ptr(shape), C.byref(g), oc.data_ptr(), n_out,
-                                       pf.data_ptr(), _ptr(pb), mf.data_ptr(), af.data_ptr(), _ptr(mb), _ptr(ab),
+                                       pf.data_ptr(), _ptr(pb), mf.data_ptr() if sort_masks else None,
+                                       af.data_ptr() if sort_masks else None, _ptr(mb), _ptr(ab),
                                        ws.data_ptr(), ws.numel(), s))
+        if not sort_masks:
+            _row_masks(pf, mf, af, s)
     return DeviceTables("strided", coords, oc, tuple(int(v) for v in shape), out_shape, pf, pb, mf, af, mb, ab)
 
 
 def stack_tables(kinds: Sequence, coords: torch.Tensor, shape: Sequence[int], rule: str = "down",
-                 stream: Optional[int] = None, backward: bool = True):
+                 stream: Optional[int] = None, backward: bool = True, sort_masks: bool = True):
     """Tables of a chain of layers; each entry of ``kinds`` goes through
     ``layer_spec``."""
     out = []
     for k in kinds:
         sp = layer_spec(k)
         if sp.kind == "subm":
-            t = subm_tables(coords, shape, stream, backward, sp.ksize, sp.dilation)
+            t = subm_tables(coords, shape, stream, backward, sp.ksize, sp.dilation, sort_masks)
         else:
-            t = strided_tables(coords, shape, rule, stream, backward, sp.ksize, sp.stride, sp.padding)
+            t = strided_tables(coords, shape, rule, stream, backward, sp.ksize, sp.stride, sp.padding, sort_masks)
         out.append(t)
         coords, shape = t.out_coords, t.out_shape
     return out

@@ -204,7 +204,10 @@ class SparseConvServer:
                                       non_blocking=True)
                 off += n
             x_dev = x32.to(B.TORCH_DT[self.dt])
-            tabs = pairs.stack_tables(self.kinds, c_dev, self.grid, stream=s.cuda_stream, backward=False)
+            # forward-only tables with masks left in row order: the mask sort
+            # would cost more than the denser tiles it saves the convolutions
+            tabs = pairs.stack_tables(self.kinds, c_dev, self.grid, stream=s.cuda_stream, backward=False,
+                                      sort_masks=False)
             st = B.ConvStack(tabs, self.channels, self.dt, self.weights, s.cuda_stream, train=False)
             st.bind(x_dev)
             st.step()

@@ -218,6 +218,16 @@ int cs_pairs_strided(const int32_t* in_coords, int64_t n_in, const int32_t* in_s
                      int64_t n_out, int32_t* pair_fwd, int32_t* pair_bwd, uint32_t* mask_fwd_sorted,
                      int32_t* argsort_fwd, uint32_t* mask_bwd_sorted, int32_t* argsort_bwd, void* workspace,
                      int64_t ws_bytes, uint64_t stream);
+/* Per-row masks in row order, for callers that pass unsorted masks to
+ * implicit_gemm2 (no counterpart in the reference, whose table generation
+ * always sorts the masks; for inference stacks, where the tile grouping does
+ * not change any row's result and skipping the sort saves its time): mask[i]
+ * word w bit b <=> pair[32w + b][i] != -1 ([n][div_up(kv, 32)] words), and
+ * identity[i] = i (the argsort to pass with them) when identity is not NULL.
+ * Queued on `stream`.  The table builders leave the forward masks to this when
+ * their mask_sorted / argsort (mask_fwd_sorted / argsort_fwd) are both NULL. */
+int cs_pairs_row_masks(const int32_t* pair, int64_t n, int32_t kv, uint32_t* mask, int32_t* identity,
+                       uint64_t stream);
 const char* cs_pairs_last_error(void);
 
 /* ---- Standalone row gather / scatter-add, the glue around matmul2 of

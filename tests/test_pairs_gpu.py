@@ -205,3 +205,53 @@ def test_device_tables_drive_a_dynamic_stack():
                      st.layers[0].dw.clone(), st.layers[2].dw.clone()])
     for a, b in zip(*outs):
         assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("kind", ["subm", "strided"])
+def test_unsorted_row_masks(kind):
+    """sort_masks=False: the same pair table, masks in row order (bit f of row
+    i <=> pair[f][i] != -1, as the host builder's masks before its sort) and
+    the identity as argsort."""
+    print(f"[pairs] unsorted row masks {kind}", flush=True)
+    grid = (16, 96, 96)
+    c = torch.from_numpy(_coords([4000, 1500], grid, 21)).cuda()
+    if kind == "subm":
+        srt = pairs.subm_tables(c, grid, backward=False)
+        uns = pairs.subm_tables(c, grid, backward=False, sort_masks=False)
+    else:
+        srt = pairs.strided_tables(c, grid, backward=False)
+        uns = pairs.strided_tables(c, grid, backward=False, sort_masks=False)
+    torch.cuda.synchronize()
+    assert torch.equal(srt.pair_fwd, uns.pair_fwd)
+    pf = _np(uns.pair_fwd)
+    exp = np.zeros(pf.shape[1], np.uint32)
+    for f in range(pf.shape[0]):
+        exp |= (pf[f] >= 0).astype(np.uint32) << np.uint32(f)
+    assert np.array_equal(_np(uns.mask_fwd).view(np.uint32), exp)
+    assert np.array_equal(_np(uns.argsort_fwd), np.arange(pf.shape[1], dtype=np.int32))
+    # the sorted masks are the row masks permuted by the argsort
+    assert np.array_equal(_np(srt.mask_fwd).view(np.uint32), exp[_np(srt.argsort_fwd)])
+
+
+def test_unsorted_masks_drive_a_stack_bit_identically():
+    """A forward-only bf16 stack (subm, strided, subm, strided) over unsorted
+    masks gives the same bits as over mask-sorted tables: a row's sum runs over
+    its own offsets in ascending order whatever tile it lands in."""
+    print("[pairs] unsorted stack", flush=True)
+    grid = (16, 96, 96)
+    c = torch.from_numpy(_coords([6000, 2500], grid, 23)).cuda()
+    kinds = ("subm", "strided", "subm", "strided")
+    chans = (16, 32, 64, 64, 128)
+    rng = np.random.default_rng(3)
+    Ws = [torch.from_numpy(rng.normal(0, 1 / np.sqrt(27 * chans[i]), size=(chans[i + 1], 27, chans[i]))
+                           .astype(np.float32)).cuda().to(torch.bfloat16) for i in range(4)]
+    x = torch.from_numpy(rng.standard_normal((c.shape[0], 16)).astype(np.float32)).cuda().to(torch.bfloat16)
+    outs = []
+    for srt in (True, False):
+        tabs = pairs.stack_tables(kinds, c, grid, backward=False, sort_masks=srt)
+        st = B.ConvStack(tabs, chans, "bf16", Ws, torch.cuda.current_stream().cuda_stream, train=False)
+        st.bind(x)
+        st.step()
+        torch.cuda.synchronize()
+        outs.append(st.layers[-1].out.clone())
+    assert torch.equal(outs[0], outs[1])
