@@ -298,7 +298,7 @@ template <typename T>
 static int ws_slices(const RowGatherArgs& a) {
   int ncol = std::min(ws_ncol<T>(a.kv, a.kred), 64);
   if (ncol == 48) ncol = 32;
-  if (a.kred == 4 * Mma<T>::kChunk && ncol > 32) ncol = 32;  // as launch_ws
+  if ((a.kred + Mma<T>::kChunk - 1) / Mma<T>::kChunk == 4 && ncol > 32) ncol = 32;  // as launch_ws
   return ncol > 0 ? (a.nout + ncol - 1) / ncol : INT_MAX;
 }
 

@@ -35,7 +35,7 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
   constexpr int E = MM::kElems;
   constexpr int CH = MM::kChunk;
   constexpr int NCOL = NBC * 16;
-  constexpr int C = KC * CH;            // reduction length handled (== p.kred)
+  constexpr int C = KC * CH;            // reduction length handled (p.kred rounded up to whole chunks)
   constexpr int CHUNKS = C / E;         // 16-byte chunks per weight row
   static_assert((CHUNKS & (CHUNKS - 1)) == 0, "chunk swizzle needs a power-of-two row");
   // offsets per gather round trip (measured on MI355X: 16 waves beat 12 waves
@@ -183,10 +183,12 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
 #pragma unroll
       for (int kc = 0; kc < KC; ++kc) {
         const int k = kc * CH + lg * E;
-        // idx < 0 sets bit 31: an offset past the extent, which reads zero.
-        // Arithmetic rather than a select keeps the load out of a branch.
+        // idx < 0 sets bit 31: an offset past the extent, which reads zero;
+        // so do reduction elements past kred (a reduction padded to whole
+        // chunks).  Arithmetic rather than a select keeps the load out of a
+        // branch.
         const uint32_t off = (uint32_t)(((int64_t)idx * p.lda + k) * (int64_t)sizeof(T)) |
-                             ((uint32_t)(idx >> 31) << 31);
+                             ((uint32_t)(idx >> 31) << 31) | ((uint32_t)(k >= p.kred) << 31);
         af[i][kc] = bld16(ra, off);
       }
     }
@@ -371,7 +373,7 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
     for (int j = 0; j < UPT; ++j) {
       const int u = tid + j * (64 * kWsWaves);
       const int f = u / (NU * KU), ku = (u / NU) % KU, nu = u % NU;
-      const bool ok = u < units && n0 + nu * E < p.nout;
+      const bool ok = u < units && n0 + nu * E < p.nout && ku * E < p.kred;  // padded reduction rows read 0
 #pragma unroll
       for (int i = 0; i < E; ++i)
         v[j][i] = bld16(rb, ok ? (uint32_t)(((int64_t)(ku * E + i) * p.b_ks + (int64_t)f * p.b_fs + n0 + nu * E) * S)
@@ -413,7 +415,7 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
       const int c = i % CHUNKS;
       const int n = (i / CHUNKS) % NCOL;
       const int f = i / (CHUNKS * NCOL);
-      const bool ok = i < total && n0 + n < p.nout;
+      const bool ok = i < total && n0 + n < p.nout && c * E < p.kred;  // padded reduction chunks read 0
       v[j] = bld16(rb, ok ? (uint32_t)(((int64_t)f * p.b_fs + (int64_t)(n0 + n) * p.b_ns + c * E) * (int)sizeof(T))
                           : kBufOOB);
     }
@@ -484,8 +486,11 @@ __global__ __launch_bounds__(64 * kWsWaves) void rowgather_ws_kernel(RowGatherAr
 template <typename T>
 inline int ws_ncol(int kv, int kred) {
   constexpr int CH = Mma<T>::kChunk;
-  if (kred % CH != 0 || kred / CH > 4 || kv < 1 || kv > 32) return 0;
-  const int64_t per_col = (int64_t)kv * kred * (int64_t)sizeof(T);
+  constexpr int E = 16 / (int)sizeof(T);
+  // a reduction of whole 16-byte rows is padded to whole MFMA chunks (the
+  // padding reads zero), e.g. 16 channels of a 16-bit type
+  if (kred < 1 || kred % E != 0 || (kred + CH - 1) / CH > 4 || kv < 1 || kv > 32) return 0;
+  const int64_t per_col = (int64_t)kv * ((kred + CH - 1) / CH * CH) * (int64_t)sizeof(T);
   int ncol = (int)(kWsLdsBytes / per_col) / 16 * 16;
   if (ncol > 64) ncol = 64;
   return ncol >= 16 ? ncol : 0;
@@ -536,7 +541,7 @@ static hipError_t launch_ws(const RowGatherArgs& a, int max_ncol, hipStream_t s,
   if (ncol == 0 || a.pair_col == nullptr || a.mask == nullptr || a.out_row != a.pair_col || a.pairs == nullptr ||
       (int64_t)a.kv * a.pair_ld * 4 >= ((int64_t)1 << 31))
     return hipSuccess;
-  const int kc = a.kred / Mma<T>::kChunk;
+  const int kc = (a.kred + Mma<T>::kChunk - 1) / Mma<T>::kChunk;
   if (kc == 4 && ncol > 32) ncol = 32;
   *used = true;
   switch (kc) {
