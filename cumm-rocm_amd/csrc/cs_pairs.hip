@@ -26,6 +26,7 @@
 
 #include <algorithm>
 
+#include "cs_device.h"
 #include "cumm_amd.h"
 
 namespace cs {
@@ -157,6 +158,22 @@ __global__ void row_masks(const int32_t* __restrict__ pair, int64_t n, int kv, u
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int W = (kv + 31) / 32;
+  if (W == 1 && (int64_t)kv * n * 4 < ((int64_t)1 << 31)) {
+    // one word: the row's kv entries load together as unconditional buffer
+    // loads (offsets f >= kv are out of range, read 0 and are masked by
+    // f < kv), so the row costs one round trip instead of one per offset
+    const __amdgpu_buffer_rsrc_t rp = make_rsrc(pair, (uint32_t)((int64_t)kv * n * 4));
+    int32_t v[32];
+#pragma unroll
+    for (int f = 0; f < 32; ++f)
+      v[f] = (int32_t)__builtin_amdgcn_raw_buffer_load_b32(
+          rp, f < kv ? (uint32_t)(((int64_t)f * n + i) * 4) : kBufOOB, 0, 0);
+    uint32_t m = 0;
+#pragma unroll
+    for (int f = 0; f < 32; ++f) m |= (uint32_t)(f < kv && v[f] >= 0) << f;
+    mask[i] = m;
+    return;
+  }
   for (int w = 0; w < W; ++w) {
     uint32_t m = 0;
     const int f1 = min(kv, 32 * w + 32);
