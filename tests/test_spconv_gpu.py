@@ -247,10 +247,10 @@ def test_wgrad_fp8_deterministic():
     assert _rel(res[0].astype(np.float64), exp) <= 1e-4
 
 
-@pytest.mark.parametrize("C,K", [(32, 16), (16, 48)])
+@pytest.mark.parametrize("C,K", [(32, 160), (16, 192)])
 def test_dgrad_block_kernel_path(C, K):
-    """dgrad whose reduction (K) is not a multiple of the f16 MFMA chunk runs
-    the block kernel, which reads the weights through a transposed copy."""
+    """dgrad whose reduction (K) is more than four f16 MFMA chunks runs the
+    block kernel, which reads the weights through a transposed copy."""
     c = subm_case(1500, C, K, seed=C * K)
     prob = c.prob
     print(f"[dgrad-block] C={C} K={K}", flush=True)
@@ -263,6 +263,32 @@ def test_dgrad_block_kernel_path(C, K):
     exp = ref.conv_dgrad(quantize(c.dO, "f16"), quantize(c.W, "f16"), prob.pair_bwd, prob.mask_sorted,
                          prob.argsort, 16, reverse_mask=True)
     assert _rel(din.double().cpu().numpy(), quantize(exp, "f16")) <= TOL["f16"]
+
+
+@pytest.mark.parametrize("dt,C,K", [("f32", 4, 16), ("f32", 12, 32), ("f16", 8, 32), ("bf16", 24, 64),
+                                    ("f16", 16, 16)])
+def test_padded_reduction_forward_dgrad(dt, C, K):
+    """Reductions of whole 16-byte rows that are not whole MFMA chunks (4 or
+    12 f32 channels, 8 / 16 / 24 16-bit channels) run the weight-stationary
+    kernel with the reduction padded by loads that read zero: forward over C
+    and dgrad over K, against the FP64 oracle."""
+    c = subm_case(1500, C, K, seed=7 * C + K)
+    prob = c.prob
+    print(f"[padded] dt={dt} C={C} K={K}", flush=True)
+    out, mo, desp = run_fwd(c, dt)
+    exp, tm = ref.conv_forward(quantize(c.X, dt), quantize(c.W, dt), prob.pair_fwd, prob.mask_sorted,
+                               prob.argsort, desp.tile_shape[0])
+    assert np.array_equal(mo, tm)
+    assert _rel(out, quantize(exp, dt)) <= TOL[dt]
+    ddesp = desp_for(1, dt, C, K)
+    din = torch.zeros((prob.n, C), dtype=TORCH_DT[dt], device="cuda")
+    p = conv_params(1, ddesp, din, to_dev(c.W, dt), to_dev(c.dO, dt), prob.pair_bwd, prob.mask_sorted,
+                    prob.argsort, reverse_mask=True)
+    ConvMainUnitTest.implicit_gemm2(p)
+    torch.cuda.synchronize()
+    exp = ref.conv_dgrad(quantize(c.dO, dt), quantize(c.W, dt), prob.pair_bwd, prob.mask_sorted, prob.argsort,
+                         16, reverse_mask=True)
+    assert _rel(din.double().cpu().numpy(), quantize(exp, dt)) <= TOL[dt]
 
 
 @pytest.mark.parametrize("dt", ["f16", "bf16"])
