@@ -394,3 +394,25 @@ def test_pair_geometry_validation():
     assert L.cs_pairs_strided_geom_workspace(1000, 500, ctypes.byref(geom(s=(0, 2, 2))), ctypes.byref(nb)) == 1
     assert L.cs_pairs_strided_out_geom_workspace(1000, ctypes.byref(geom(p=(-1, 0, 0))), 0, ctypes.byref(nb)) == 1
     assert L.cs_pairs_subm_geom_workspace(1000, None, ctypes.byref(nb)) == 1
+
+
+def test_row_masks_and_unsorted_tables_validation():
+    """cs_pairs_row_masks checks its arguments before any device work (n = 0
+    is a no-op), and the table builders take the forward mask outputs both
+    set or both null."""
+    L = _abi.lib()
+    assert L.cs_pairs_row_masks(None, -1, 27, None, None, 0) == 1
+    assert b"bad argument" in L.cs_pairs_last_error()
+    assert L.cs_pairs_row_masks(None, 10, 0, None, None, 0) == 1
+    assert L.cs_pairs_row_masks(None, 10, 1025, None, None, 0) == 1
+    assert L.cs_pairs_row_masks(None, 10, 27, None, None, 0) == 1  # rows but no pair table / mask buffer
+    assert L.cs_pairs_row_masks(None, 0, 27, None, None, 0) == 0
+    g = _abi.CsConvGeom()
+    for i in range(3):
+        g.ksize[i], g.stride[i], g.padding[i], g.dilation[i] = 3, 1, 0, 1
+    shape = (ctypes.c_int32 * 3)(16, 96, 96)
+    # a mask buffer without an argsort (or the reverse) is refused before any
+    # buffer is touched: the non-null values below are never dereferenced
+    assert L.cs_pairs_subm_geom(None, 10, shape, ctypes.byref(g), None, None, 64, None, None, 0, 0) == 1
+    assert b"both set or both null" in L.cs_pairs_last_error()
+    assert L.cs_pairs_subm_geom(None, 10, shape, ctypes.byref(g), None, None, None, 64, None, 0, 0) == 1
