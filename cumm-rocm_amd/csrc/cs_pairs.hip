@@ -133,6 +133,29 @@ __global__ void subm_lookup(const int32_t* __restrict__ coords, int64_t n, Shape
   if (pair_bwd) pair_bwd[(int64_t)(g.kv() - 1 - f) * n + i] = j;
 }
 
+// Forward-only submanifold tables with half the lookups.  For a centred odd
+// kernel the neighbour relation is symmetric: coord_i + d_f = coord_j <=>
+// coord_j + d_{kv-1-f} = coord_i.  Thread (f, i), f <= kv/2, looks up
+// coord_i + d_f and writes pair[f][i] = j and, for f < kv/2 and j found,
+// pair[kv-1-f][j] = i (one writer per entry for distinct coordinates); the
+// rows f > kv/2 hold -1 beforehand, which rows nobody reaches keep.
+__global__ void subm_lookup_half(const int32_t* __restrict__ coords, int64_t n, Shape3 s, Geom g,
+                                 const uint64_t* tkeys, const int32_t* tvals, uint64_t tmask, int32_t* pair_fwd) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = blockIdx.y;
+  if (i >= n) return;
+  const int32_t* c = coords + 4 * i;
+  int kz, ky, kx;
+  offset_of(g, f, kz, ky, kx);
+  const int64_t z = c[1] + (int64_t)g.d0 * (kz - g.k0 / 2), y = c[2] + (int64_t)g.d1 * (ky - g.k1 / 2),
+                x = c[3] + (int64_t)g.d2 * (kx - g.k2 / 2);
+  int32_t j = -1;
+  if (z >= 0 && z < s.s0 && y >= 0 && y < s.s1 && x >= 0 && x < s.s2) j = hash_find(key_of(c[0], z, y, x, s), tkeys, tvals, tmask);
+  pair_fwd[(int64_t)f * n + i] = j;
+  const int kv = g.kv();
+  if (f < kv / 2 && j >= 0 && j < n) pair_fwd[(int64_t)(kv - 1 - f) * n + j] = (int32_t)i;
+}
+
 // Strided: thread (f, o) looks up o*stride - padding + k_f among the inputs;
 // the inverse entry goes to pair_bwd[f][i] (when pair_bwd is set).
 __global__ void strided_lookup(const int32_t* __restrict__ out_coords, int64_t n_out, int64_t n_in, Shape3 in_s,
@@ -355,8 +378,20 @@ int subm_impl(const int32_t* coords, int64_t n, const int32_t* shape, const Geom
   hipLaunchKernelGGL(hash_insert, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, coords, n, sh, tkeys, tvals,
                      cap - 1);
   PCHK(hipGetLastError());
-  hipLaunchKernelGGL(subm_lookup, dim3((unsigned)((n + 255) / 256), (unsigned)g.kv()), dim3(256), 0, s, coords, n,
-                     sh, g, tkeys, tvals, cap - 1, pair_fwd, pair_bwd);
+  if (pair_bwd == nullptr) {
+    // forward-only: half the lookups, the mirrored offsets by symmetry
+    const int ctr = g.kv() / 2;
+    const int64_t upper = (int64_t)(g.kv() - 1 - ctr) * n;
+    if (upper > 0) {
+      hipLaunchKernelGGL(fill_i32, grid1(upper), dim3(256), 0, s, pair_fwd + (int64_t)(ctr + 1) * n, upper, -1);
+      PCHK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(subm_lookup_half, dim3((unsigned)((n + 255) / 256), (unsigned)(ctr + 1)), dim3(256), 0, s,
+                       coords, n, sh, g, tkeys, tvals, cap - 1, pair_fwd);
+  } else {
+    hipLaunchKernelGGL(subm_lookup, dim3((unsigned)((n + 255) / 256), (unsigned)g.kv()), dim3(256), 0, s, coords,
+                       n, sh, g, tkeys, tvals, cap - 1, pair_fwd, pair_bwd);
+  }
   PCHK(hipGetLastError());
   if (mask_sorted == nullptr) return 0;  // pair tables only (row masks from cs_pairs_row_masks)
   return sort_masks(pair_fwd, n, g.kv(), mask_sorted, argsort, cv, s);

@@ -106,6 +106,27 @@ def test_forward_only_tables(kind):
         B.DeviceLayer(fwd, 16, 16, "f16", np.zeros((16, 27, 16), np.float32), 0)
 
 
+@pytest.mark.parametrize("ksize,dilation", [((5, 5, 5), (1, 1, 1)), ((3, 5, 3), (2, 1, 3)), ((1, 1, 1), (1, 1, 1)),
+                                            ((7, 3, 1), (1, 2, 1))])
+def test_forward_only_subm_kernel_shapes(ksize, dilation):
+    """Forward-only submanifold tables look up half the offsets and fill the
+    mirrored ones by symmetry; the result equals the full build's forward table
+    and the host builder's for anisotropic and dilated kernels."""
+    print(f"[pairs] forward-only subm {ksize} dil {dilation}", flush=True)
+    grid = (16, 96, 96)
+    c = _coords([3000, 900], grid, 33)
+    dc = torch.from_numpy(c).cuda()
+    full = pairs.subm_tables(dc, grid, ksize=ksize, dilation=dilation)
+    fwd = pairs.subm_tables(dc, grid, backward=False, ksize=ksize, dilation=dilation)
+    uns = pairs.subm_tables(dc, grid, backward=False, ksize=ksize, dilation=dilation, sort_masks=False)
+    torch.cuda.synchronize()
+    for name in ("pair_fwd", "mask_fwd", "argsort_fwd"):
+        assert torch.equal(getattr(full, name), getattr(fwd, name)), name
+    assert torch.equal(full.pair_fwd, uns.pair_fwd)
+    host = pc.build_subm(c, grid, ksize=ksize, dilation=dilation)
+    np.testing.assert_array_equal(fwd.pair_fwd.cpu().numpy(), host.pair_fwd)
+
+
 @pytest.mark.parametrize("kind", ["subm", "strided"])
 def test_tables_on_a_side_stream(kind):
     """Tables built on a stream other than the current one: scratch and outputs
