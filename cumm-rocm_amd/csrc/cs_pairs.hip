@@ -17,7 +17,9 @@
 //
 // Kernels: hash insert (open addressing, linear probing, 64-bit CAS), one
 // thread per (offset, row) lookup writing pair_fwd / pair_bwd coalesced along
-// rows, a per-row mask kernel, and rocPRIM radix sorts (stable LSD, one pass
+// rows (submanifold: a lookup per output row; strided: the outputs are
+// hashed and each input finds the output it reaches per offset), a per-row
+// mask kernel, and rocPRIM radix sorts (stable LSD, one pass
 // per mask word, least significant word first) for the mask argsort and for
 // the unique output coordinates of strided layers.
 #include <hip/hip_runtime.h>
@@ -83,11 +85,14 @@ __global__ void iota_i32(int32_t* p, int64_t n) {
     p[i] = (int32_t)i;
 }
 
+// bounded: rows outside the shape are left out (their keys would alias
+// other cells)
 __global__ void hash_insert(const int32_t* __restrict__ coords, int64_t n, Shape3 s, uint64_t* tkeys,
-                            int32_t* tvals, uint64_t tmask) {
+                            int32_t* tvals, uint64_t tmask, int bounded) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int32_t* c = coords + 4 * i;
+  if (bounded && (c[1] < 0 || c[1] >= s.s0 || c[2] < 0 || c[2] >= s.s1 || c[3] < 0 || c[3] >= s.s2)) return;
   const uint64_t key = key_of(c[0], c[1], c[2], c[3], s);
   uint64_t h = mix64(key) & tmask;
   for (uint64_t probe = 0; probe <= tmask; ++probe) {
@@ -156,24 +161,33 @@ __global__ void subm_lookup_half(const int32_t* __restrict__ coords, int64_t n, 
   if (f < kv / 2 && j >= 0 && j < n) pair_fwd[(int64_t)(kv - 1 - f) * n + j] = (int32_t)i;
 }
 
-// Strided: thread (f, o) looks up o*stride - padding + k_f among the inputs;
-// the inverse entry goes to pair_bwd[f][i] (when pair_bwd is set).
-__global__ void strided_lookup(const int32_t* __restrict__ out_coords, int64_t n_out, int64_t n_in, Shape3 in_s,
-                               Geom g, const uint64_t* tkeys, const int32_t* tvals, uint64_t tmask,
-                               int32_t* pair_fwd, int32_t* pair_bwd) {
-  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Strided, from the input side: thread (f, i) finds the one output o with
+// o*stride - padding + k_f = coord_i (where the stride divides and o lies in
+// the output grid) and looks o up among the outputs: pair_fwd[f][o] = i and
+// pair_bwd[f][i] = o, or -1.  An input reaches at most one output per offset,
+// so each pair_fwd entry has at most one writer (distinct coordinates), and
+// the entries no input reaches keep the -1 written before.  About
+// kv / stride^3 hash probes per input, where a lookup per output row would
+// make kv of them.  Output coordinates outside the output grid get no pairs.
+__global__ void strided_scatter(const int32_t* __restrict__ in_coords, int64_t n_in, Shape3 in_s, Shape3 out_s,
+                                Geom g, const uint64_t* tkeys, const int32_t* tvals, uint64_t tmask, int64_t n_out,
+                                int32_t* pair_fwd, int32_t* pair_bwd) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int f = blockIdx.y;
-  if (o >= n_out) return;
-  const int32_t* c = out_coords + 4 * o;
+  if (i >= n_in) return;
+  const int32_t* c = in_coords + 4 * i;
   int kz, ky, kx;
   offset_of(g, f, kz, ky, kx);
-  const int64_t z = (int64_t)g.st0 * c[1] - g.p0 + kz, y = (int64_t)g.st1 * c[2] - g.p1 + ky,
-                x = (int64_t)g.st2 * c[3] - g.p2 + kx;
-  int32_t i = -1;
-  if (z >= 0 && z < in_s.s0 && y >= 0 && y < in_s.s1 && x >= 0 && x < in_s.s2)
-    i = hash_find(key_of(c[0], z, y, x, in_s), tkeys, tvals, tmask);
-  pair_fwd[(int64_t)f * n_out + o] = i;
-  if (pair_bwd != nullptr && i >= 0 && i < n_in) pair_bwd[(int64_t)f * n_in + i] = (int32_t)o;
+  const int64_t nz = (int64_t)c[1] + g.p0 - kz, ny = (int64_t)c[2] + g.p1 - ky, nx = (int64_t)c[3] + g.p2 - kx;
+  int32_t o = -1;
+  const bool inside = c[1] >= 0 && c[1] < in_s.s0 && c[2] >= 0 && c[2] < in_s.s1 && c[3] >= 0 && c[3] < in_s.s2;
+  if (inside && nz >= 0 && ny >= 0 && nx >= 0 && nz % g.st0 == 0 && ny % g.st1 == 0 && nx % g.st2 == 0) {
+    const int64_t z = nz / g.st0, y = ny / g.st1, x = nx / g.st2;
+    if (z < out_s.s0 && y < out_s.s1 && x < out_s.s2) o = hash_find(key_of(c[0], z, y, x, out_s), tkeys, tvals, tmask);
+  }
+  if (o >= n_out) o = -1;
+  if (o >= 0) pair_fwd[(int64_t)f * n_out + o] = (int32_t)i;
+  if (pair_bwd != nullptr) pair_bwd[(int64_t)f * n_in + i] = o;
 }
 
 // mask[i][w] bit b <=> pair[32w + b][i] >= 0 (W words per row)
@@ -376,7 +390,7 @@ int subm_impl(const int32_t* coords, int64_t n, const int32_t* shape, const Geom
   hipLaunchKernelGGL(fill_u64, grid1((int64_t)cap), dim3(256), 0, s, tkeys, (int64_t)cap, kEmptyKey);
   PCHK(hipGetLastError());
   hipLaunchKernelGGL(hash_insert, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, coords, n, sh, tkeys, tvals,
-                     cap - 1);
+                     cap - 1, 0);
   PCHK(hipGetLastError());
   if (pair_bwd == nullptr) {
     // forward-only: half the lookups, the mirrored offsets by symmetry
@@ -446,27 +460,30 @@ int strided_impl(const int32_t* in_coords, int64_t n_in, const int32_t* in_shape
                  int32_t* afs, uint32_t* mbs, int32_t* abs_, Carver& cv, hipStream_t s) {
   const int kv = g.kv();
   const Shape3 is{in_shape[0], in_shape[1], in_shape[2]};
-  const uint64_t cap = table_cap(n_in);
+  // the hash holds the outputs (strided_scatter); sized for either side
+  const uint64_t cap = table_cap(std::max(n_in, n_out));
   uint64_t* tkeys = cv.take<uint64_t>((int64_t)cap);
   int32_t* tvals = cv.take<int32_t>((int64_t)cap);
   if (!cv.base) {
     sort_masks(nullptr, std::max(n_in, n_out), kv, nullptr, nullptr, cv, s);
     return 0;
   }
-  if (n_in > 0) {
+  const Shape3 os = out_shape_of(in_shape, g);
+  if (n_out > 0) {
     hipLaunchKernelGGL(fill_u64, grid1((int64_t)cap), dim3(256), 0, s, tkeys, (int64_t)cap, kEmptyKey);
     PCHK(hipGetLastError());
-    hipLaunchKernelGGL(hash_insert, dim3((unsigned)((n_in + 255) / 256)), dim3(256), 0, s, in_coords, n_in, is,
-                       tkeys, tvals, cap - 1);
+    hipLaunchKernelGGL(hash_insert, dim3((unsigned)((n_out + 255) / 256)), dim3(256), 0, s, out_coords, n_out, os,
+                       tkeys, tvals, cap - 1, 1);
     PCHK(hipGetLastError());
-    if (pair_bwd != nullptr) {
-      hipLaunchKernelGGL(fill_i32, grid1(kv * n_in), dim3(256), 0, s, pair_bwd, kv * n_in, -1);
-      PCHK(hipGetLastError());
-    }
+    hipLaunchKernelGGL(fill_i32, grid1(kv * n_out), dim3(256), 0, s, pair_fwd, kv * n_out, -1);
+    PCHK(hipGetLastError());
   }
-  if (n_out > 0) {
-    hipLaunchKernelGGL(strided_lookup, dim3((unsigned)((n_out + 255) / 256), (unsigned)kv), dim3(256), 0, s,
-                       out_coords, n_out, n_in, is, g, tkeys, tvals, cap - 1, pair_fwd, pair_bwd);
+  if (n_in > 0 && n_out > 0) {
+    hipLaunchKernelGGL(strided_scatter, dim3((unsigned)((n_in + 255) / 256), (unsigned)kv), dim3(256), 0, s,
+                       in_coords, n_in, is, os, g, tkeys, tvals, cap - 1, n_out, pair_fwd, pair_bwd);
+    PCHK(hipGetLastError());
+  } else if (n_in > 0 && pair_bwd != nullptr) {
+    hipLaunchKernelGGL(fill_i32, grid1(kv * n_in), dim3(256), 0, s, pair_bwd, kv * n_in, -1);
     PCHK(hipGetLastError());
   }
   const int64_t mark = cv.off;
