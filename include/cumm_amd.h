@@ -212,7 +212,10 @@ int cs_pairs_strided_out(const int32_t* coords, int64_t n, const int32_t* in_sha
  * (output rows), forward masks over the outputs and backward masks over the
  * inputs, each sorted with its argsort.  Forward-only callers (inference)
  * pass pair_bwd, mask_bwd_sorted and argsort_bwd all NULL; cs_pairs_subm
- * likewise accepts pair_bwd = NULL. */
+ * likewise accepts pair_bwd = NULL.  out_coords may be any list of distinct
+ * outputs in any order (rows outside the output grid
+ * (S + 2*padding - k) / stride + 1 get no pairs); input and submanifold
+ * coordinates must be distinct. */
 int cs_pairs_strided_workspace(int64_t n_in, int64_t n_out, int64_t* nbytes);
 int cs_pairs_strided(const int32_t* in_coords, int64_t n_in, const int32_t* in_shape, const int32_t* out_coords,
                      int64_t n_out, int32_t* pair_fwd, int32_t* pair_bwd, uint32_t* mask_fwd_sorted,

@@ -201,6 +201,47 @@ def test_strided_geometry_tables_match_host(ksize, stride, padding, rule):
     assert np.array_equal(_np(dev.argsort_bwd), host.argsort_bwd)
 
 
+def test_strided_tables_for_a_given_output_list():
+    """cs_pairs_strided_geom over an output list the caller chose (a shuffled
+    subset of the outputs plus one row outside the output grid): listed outputs
+    get the host builder's pairs, unlisted outputs leave their inputs' pair_bwd
+    entries at -1, and the out-of-grid row gets no pairs."""
+    import ctypes as C
+    from cumm_amd import _abi
+    print("[pairs] strided over a given output list", flush=True)
+    grid = (16, 96, 96)
+    c = _coords([2500, 700], grid, 41)
+    oc, oshape = pc.strided_out_coords(c, grid, (3, 3, 3), np.array([2, 2, 2]), np.array([1, 1, 1]), "reach")
+    rng = np.random.default_rng(3)
+    keep = np.sort(rng.choice(len(oc), len(oc) // 2, replace=False))
+    sub = oc[keep][rng.permutation(len(keep))]
+    sub = np.concatenate([sub, np.array([[0, oshape[0], 3, 3]], np.int32)])  # z one past the grid
+    hf, hb = pc.strided_pairs(c, grid, sub[:-1], (3, 3, 3), np.array([2, 2, 2]), np.array([1, 1, 1]))
+    L = _abi.lib()
+    g = pairs.LayerSpec("strided").cs()
+    n_in, n_out, kv = len(c), len(sub), 27
+    dc, do = torch.from_numpy(c).cuda(), torch.from_numpy(np.ascontiguousarray(sub)).cuda()
+    pf = torch.empty((kv, n_out), dtype=torch.int32, device="cuda")
+    pb = torch.empty((kv, n_in), dtype=torch.int32, device="cuda")
+    mf = torch.empty((n_out,), dtype=torch.int32, device="cuda")
+    af = torch.empty((n_out,), dtype=torch.int32, device="cuda")
+    mb = torch.empty((n_in,), dtype=torch.int32, device="cuda")
+    ab = torch.empty((n_in,), dtype=torch.int32, device="cuda")
+    nb = C.c_int64(0)
+    assert L.cs_pairs_strided_geom_workspace(n_in, n_out, C.byref(g), C.byref(nb)) == 0
+    ws = torch.empty((max(nb.value, 1),), dtype=torch.uint8, device="cuda")
+    s = torch.cuda.current_stream().cuda_stream
+    assert L.cs_pairs_strided_geom(dc.data_ptr(), n_in, pairs._shape_ptr(grid), C.byref(g), do.data_ptr(), n_out,
+                                   pf.data_ptr(), pb.data_ptr(), mf.data_ptr(), af.data_ptr(), mb.data_ptr(),
+                                   ab.data_ptr(), ws.data_ptr(), ws.numel(), s) == 0
+    torch.cuda.synchronize()
+    got_f, got_b = _np(pf), _np(pb)
+    np.testing.assert_array_equal(got_f[:, :-1], hf)
+    assert (got_f[:, -1] == -1).all()
+    np.testing.assert_array_equal(got_b, hb)
+    assert (got_b >= 0).any() and (got_b == -1).any()
+
+
 def test_device_tables_drive_a_dynamic_stack():
     """5x5x5 submanifold layers (dynamic masks) and a 2x2x2 stride-2 layer,
     device-resident: the stack gives the same bits over device and host tables."""
